@@ -1,0 +1,213 @@
+/*
+ * ipgpu.h -- C ABI of libipgpu.so, the MI355X (gfx950) force-field engine
+ * behind InteratomicPotentials.jl's neighbor-list, pair-potential and SNAP
+ * calls.
+ *
+ * Conventions
+ *  - Every entry point returns an int status: IPG_OK (0) or a negative
+ *    IPG_ERR_* code.  ipg_last_error() returns a thread-local message for the
+ *    last failure on the calling thread.  No C++ exception or abort crosses
+ *    the ABI.
+ *  - Host-pointer entry points copy inputs to the device, run on the
+ *    context's HIP stream and copy results back before returning (they are
+ *    synchronous).  The caller owns all host buffers; the library keeps no
+ *    host pointer after return.
+ *  - *_dev entry points take device pointers for every array (inputs and
+ *    outputs), enqueue work on the context's stream and return without
+ *    synchronizing; ipg_ctx_synchronize() waits for them.
+ *  - Atom indices are 0-based (the Julia wrapper adds 1).
+ *  - Positions are AoS x,y,z per atom (Julia Vector{SVector{3,Float64}}
+ *    reinterpreted).  Boxes are orthorhombic: box[d] is bounding_box(s)[d][d].
+ *  - Calls on one context are serialized by the caller; different contexts
+ *    may be used from different threads.  Reductions are deterministic:
+ *    repeating a call gives bit-identical results.
+ */
+#ifndef IPGPU_H
+#define IPGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IPG_VERSION 100
+
+/* status codes */
+#define IPG_OK 0
+#define IPG_ERR_INVALID (-1)     /* invalid argument (n < 0, rcut <= 0, NaN, bad box, ...) */
+#define IPG_ERR_CAPACITY (-2)    /* output capacity too small; *npairs holds the size needed */
+#define IPG_ERR_HIP (-3)         /* HIP runtime error */
+#define IPG_ERR_UNSUPPORTED (-4) /* option not implemented on the GPU (e.g. chem_flag) */
+#define IPG_ERR_NOMEM (-5)       /* device allocation failed */
+
+typedef struct ipg_ctx ipg_ctx;
+typedef struct ipg_nlist ipg_nlist;
+typedef struct ipg_snap ipg_snap;
+
+/* ------------------------------------------------------------------ */
+/* context, errors, device memory, profiling                          */
+/* ------------------------------------------------------------------ */
+int ipg_get_version(void);
+const char* ipg_last_error(void);
+int ipg_ctx_create(int device, ipg_ctx** out);
+int ipg_ctx_destroy(ipg_ctx* ctx);
+int ipg_ctx_synchronize(ipg_ctx* ctx);
+int ipg_device_malloc(ipg_ctx* ctx, size_t bytes, void** dptr);
+int ipg_device_free(ipg_ctx* ctx, void* dptr);
+int ipg_memcpy_h2d(ipg_ctx* ctx, void* dst, const void* src, size_t bytes);
+int ipg_memcpy_d2h(ipg_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* Per-kernel timing with HIP events recorded on the context stream around
+ * every launch (off by default). */
+int ipg_prof_enable(ipg_ctx* ctx, int on);
+int ipg_prof_reset(ipg_ctx* ctx);
+/* Accumulated launches and milliseconds of the kernel named `kernel`
+ * (flushes pending events, synchronizing the stream). */
+int ipg_prof_query(ipg_ctx* ctx, const char* kernel, int64_t* launches, double* total_ms);
+
+/* ------------------------------------------------------------------ */
+/* systems (AtomsBase AbstractSystem, reduced to what the path reads)  */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  int64_t natoms;
+  const double* positions; /* 3*natoms; host or device pointer per entry point */
+  const int32_t* species;  /* natoms species indices into the potential's table; NULL = all 0 */
+  double box[3];           /* box edge lengths (only read for periodic dimensions) */
+  int32_t periodic[3];     /* periodicity(s): nonzero = periodic */
+} ipg_system;
+
+/* ------------------------------------------------------------------ */
+/* neighbor lists                                                      */
+/* ------------------------------------------------------------------ */
+enum { IPG_LIST_HALF = 0, IPG_LIST_FULL = 1 };
+
+/* Replaces src/nnlist.jl:23-46 neighborlist(s, rcutoff) (half list, j > i)
+ * and its full (j != i) variant.  Membership is the reference's:
+ * PeriodicEuclidean(L) distance <= rcut.  disp[q] = get_displacement(L,
+ * x_i, x_j) (src/nnlist.jl:18-21).  offsets has natoms+1 entries; j are
+ * sorted per atom.  If cap < the pair count, returns IPG_ERR_CAPACITY with
+ * *npairs = count (offsets are still written). disp may be NULL. */
+int ipg_neighborlist(ipg_ctx* ctx, const ipg_system* sys, double rcut, int kind, int64_t* offsets, int32_t* nbr,
+                     double* disp, int64_t cap, int64_t* npairs);
+
+/* Device-resident list for repeated evaluations (sys holds device pointers). */
+int ipg_nlist_build_dev(ipg_ctx* ctx, const ipg_system* sys, double rcut, int kind, ipg_nlist** out);
+int ipg_nlist_info(const ipg_nlist* list, int64_t* natoms, int64_t* npairs, const int64_t** d_offsets,
+                   const int32_t** d_nbr);
+int ipg_nlist_destroy(ipg_nlist* list);
+
+/* ------------------------------------------------------------------ */
+/* pair potentials                                                     */
+/* ------------------------------------------------------------------ */
+enum {
+  IPG_PAIR_LJ = 0,          /* p = {eps, sigma}           src/empirical/lj.jl:35-36      */
+  IPG_PAIR_MORSE = 1,       /* p = {D, alpha, sigma}      src/empirical/morse.jl:36-39   */
+  IPG_PAIR_ZBL = 2,         /* p = {Z1, Z2, e}            src/empirical/zbl.jl:27-36     */
+  IPG_PAIR_BORN_MAYER = 3,  /* p = {A, rho, sigma, C, D}  src/empirical/bm.jl:38-39      */
+  IPG_PAIR_COULOMB = 4,     /* p = {q1, q2}               src/empirical/coulomb.jl:36-37 */
+  IPG_PAIR_MOCK = 5         /* p = {seed}: V = R^2 seed   test/mocks.jl:22-34            */
+};
+enum { IPG_FP64 = 0, IPG_FP32 = 1 };
+
+typedef struct {
+  int32_t kind;
+  int32_t reserved;
+  double coeff;          /* LinearCombinationPotential coefficient (1 for a plain potential) */
+  double rcut;           /* get_rcutoff(p), atomic units */
+  double p[6];           /* parameters, atomic units (austrip'ed as the constructors do) */
+  uint64_t species_mask; /* bit s set <=> species s in get_species(p); all ones = missing */
+} ipg_pair_term;
+
+/* Replaces energy_and_force / virial_stress (src/types/empirical_potential.jl
+ * :49-67) and their LinearCombinationPotential sums
+ * (src/mixed/linear_combination_potential.jl:27-39), fused: one list at the
+ * largest rcut, every term evaluated per pair with its own rcut and species
+ * filter.  Energy in hartree, forces hartree/bohr (3*natoms), virial in
+ * Voigt order [xx,yy,zz,zy,zx,yx]; any output may be NULL.  precision
+ * IPG_FP32 evaluates the pair terms in float with double reductions. */
+int ipg_pair_energy_forces(ipg_ctx* ctx, const ipg_system* sys, const ipg_pair_term* terms, int nterms,
+                           int precision, double* energy, double* forces, double* virial);
+/* Device variant: every pointer (sys arrays, energy[1], forces[3N],
+ * virial[6]) is a device pointer; terms is a host array. */
+int ipg_pair_energy_forces_dev(ipg_ctx* ctx, const ipg_system* sys, const ipg_pair_term* terms, int nterms,
+                               int precision, double* energy, double* forces, double* virial);
+/* Pair kernel alone over a prebuilt device full list (IPG_LIST_FULL at a
+ * cutoff >= every term's rcut); pointers as in the _dev variant. */
+int ipg_pair_eval_dev(ipg_ctx* ctx, const ipg_nlist* full_list, const ipg_system* sys, const ipg_pair_term* terms,
+                      int nterms, int precision, double* energy, double* forces, double* virial);
+
+/* ------------------------------------------------------------------ */
+/* SNAP                                                                */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  int32_t twojmax;
+  int32_t nspecies;
+  double rcutfac, rmin0, rfac0;
+  const double* radii;  /* host, nspecies */
+  const double* weight; /* host, nspecies */
+  int32_t chem_flag, bzero_flag, bnorm_flag, switch_flag, wselfall_flag;
+} ipg_snap_params;
+
+/* SNAP neighbor-list modes.  REFERENCE reproduces utl.jl:38-101 as-is
+ * (inrange radius sqrt(cutmax) :81, element lookup by first appearance :71
+ * and by enumerate index :86).  CUTOFF queries radius cutmax and uses each
+ * neighbor's own species. */
+enum { IPG_SNAP_LIST_REFERENCE = 0, IPG_SNAP_LIST_CUTOFF = 1 };
+enum { IPG_SNAP_PLAIN_FP64 = 0, IPG_SNAP_ADJOINT_FP64 = 1, IPG_SNAP_ADJOINT_FP32 = 2 };
+
+/* Replaces SNAP(...) construction + initialize_prebuilt_arrays
+ * (src/BasisSystems/SNAP/snap.jl:20-36, types/prebuilt_arrays.jl:48-321):
+ * builds the index, Clebsch-Gordan and rootpq tables in FP64 and uploads
+ * them.  twojmax must be in [0, 8]; chem_flag must be 0 on the GPU. */
+int ipg_snap_create(ipg_ctx* ctx, const ipg_snap_params* params, ipg_snap** out);
+int ipg_snap_destroy(ipg_snap* snap);
+/* ncoeff = get_num_snap_coeffs (utl.jl:19-37); table sizes. */
+int ipg_snap_info(const ipg_snap* snap, int32_t* ncoeff, int32_t* idxu_max, int32_t* idxz_max, int32_t* idxb_max);
+
+/* Replaces neighborlist(A, snap) (utl.jl:38-101): full list, sorted j,
+ * R = |r|, r = x_j - x_i (min image).  appear (may be NULL = species) gives
+ * each atom's index in unique(atomic_symbol(A)), used by REFERENCE mode. */
+int ipg_snap_neighborlist(ipg_snap* snap, const ipg_system* sys, int mode, const int32_t* appear, int64_t* offsets,
+                          int32_t* nbr, double* R, double* r, int64_t cap, int64_t* npairs);
+
+/* Replaces compute_local_descriptors (snap.jl:103-124): B natoms x ncoeff. */
+int ipg_snap_local_descriptors(ipg_snap* snap, const ipg_system* sys, int mode, const int32_t* appear, double* B);
+/* Replaces compute_force_descriptors (snap.jl:126-167) with the row fix of
+ * compute_all_descriptors (snap.jl:261): dB natoms x (ncoeff*nspecies) x 3,
+ * dB[k] = -d(sum_i B_i)/dx_k in the block of the species of i. */
+int ipg_snap_force_descriptors(ipg_snap* snap, const ipg_system* sys, int mode, const int32_t* appear, double* dB);
+/* Replaces compute_all_descriptors (snap.jl:226-285): B, dB as above and
+ * W (ncoeff*nspecies) x 6 summed over atoms, rows [x dBx, y dBy, z dBz,
+ * z dBy, z dBx, y dBx] with raw positions (:267-281).  Outputs may be NULL. */
+int ipg_snap_all_descriptors(ipg_snap* snap, const ipg_system* sys, int mode, const int32_t* appear, double* B,
+                             double* dB, double* W);
+/* Replaces energy_and_force(s, LBasisPotential(beta, beta0, snap))
+ * (src/BasisPotentials/basis_potentials.jl:32-41, linear_potential.jl:20-35):
+ * E = beta0 + sum_i beta_{s(i)} . B_i, F_k = sum_c beta_c dB[k][:, c].
+ * nbeta = ncoeff (one beta for every species, as the reference) or
+ * ncoeff*nspecies (one block per species).  path selects the plain FP64
+ * route (dB then beta, as the reference does) or the adjoint route
+ * (Y = dE/dU, no per-pair dB) in FP64 or FP32 (FP64 reductions).  forces
+ * 3*natoms; either output may be NULL. */
+int ipg_snap_energy_forces(ipg_snap* snap, const ipg_system* sys, int mode, const int32_t* appear,
+                           const double* beta, int64_t nbeta, double beta0, int path, double* energy,
+                           double* forces);
+/* Device variant: sys arrays, beta (ncoeff*nspecies, expanded), energy[1]
+ * and forces[3N] are device pointers. */
+int ipg_snap_energy_forces_dev(ipg_snap* snap, const ipg_system* sys, int mode, const double* beta, double beta0,
+                               int path, double* energy, double* forces);
+/* Reverse mode of F(x, beta) for force matching (a ChainRulesCore rrule of
+ * the force call): given the cotangent gF = dL/dF (3*natoms), returns
+ * gbeta = gF^T dF/dbeta (nbeta entries, same layout as beta) and
+ * gpos = gF^T dF/dx (3*natoms), exactly (forward-mode tangents through the
+ * adjoint, no finite differences).  precision IPG_FP64 or IPG_FP32.  Either
+ * output may be NULL.  This has no counterpart in the reference (its NN/
+ * fitting stack differentiates through Zygote on the CPU). */
+int ipg_snap_force_vjp(ipg_snap* snap, const ipg_system* sys, int mode, const int32_t* appear, const double* beta,
+                       int64_t nbeta, const double* gF, int precision, double* gbeta, double* gpos);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

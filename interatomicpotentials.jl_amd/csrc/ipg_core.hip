@@ -1,0 +1,387 @@
+// ipg_core.hip -- context, errors, device memory, profiling, scans and
+// deterministic reductions shared by every entry point.
+#include "ipg_internal.h"
+
+#include <cfloat>
+
+namespace ipg {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+void* scratch(ipg_ctx* ctx, const char* name, size_t bytes) {
+  auto& b = ctx->scratch[name];
+  if (b.bytes >= bytes && b.ptr) return b.ptr;
+  if (b.ptr) {
+    // the old buffer may still be read by queued kernels
+    IPG_HIP(hipStreamSynchronize(ctx->stream));
+    IPG_HIP(hipFree(b.ptr));
+    b.ptr = nullptr;
+    b.bytes = 0;
+  }
+  size_t want = bytes < 256 ? 256 : bytes;
+  want = want + want / 4;  // headroom for slowly growing inputs
+  if (hipMalloc(&b.ptr, want) != hipSuccess) {
+    (void)hipGetLastError();
+    b.ptr = nullptr;
+    fail(IPG_ERR_NOMEM, std::string("device allocation of ") + std::to_string(want) + " bytes failed (" + name + ")");
+  }
+  b.bytes = want;
+  return b.ptr;
+}
+
+hipEvent_t get_event(ipg_ctx* ctx) {
+  if (!ctx->free_events.empty()) {
+    hipEvent_t e = ctx->free_events.back();
+    ctx->free_events.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  IPG_HIP(hipEventCreate(&e));
+  return e;
+}
+
+void flush_prof(ipg_ctx* ctx) {
+  if (ctx->pending.empty()) return;
+  IPG_HIP(hipStreamSynchronize(ctx->stream));
+  for (auto& p : ctx->pending) {
+    float ms = 0.f;
+    IPG_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+    auto& s = ctx->stats[p.name];
+    s.first += 1;
+    s.second += (double)ms;
+    ctx->free_events.push_back(p.a);
+    ctx->free_events.push_back(p.b);
+  }
+  ctx->pending.clear();
+}
+
+void h2d(ipg_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return;
+  IPG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+}
+void d2h(ipg_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return;
+  IPG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+}
+void sync(ipg_ctx* ctx) { IPG_HIP(hipStreamSynchronize(ctx->stream)); }
+
+__global__ void k_fill_zero(uint32_t* __restrict__ p, int64_t nwords) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < nwords; i += stride) p[i] = 0u;
+}
+
+// Zero-fill with a kernel (no hipMemsetAsync, so the call stays capturable
+// in a HIP graph).  bytes must be a multiple of 4.
+void zero(ipg_ctx* ctx, void* dst, size_t bytes) {
+  if (bytes == 0) return;
+  int64_t nw = (int64_t)(bytes / 4);
+  unsigned g = grid_for(nw, 256);
+  if (g > 4096) g = 4096;
+  launch(ctx, "fill_zero", [&] { k_fill_zero<<<g, 256, 0, ctx->stream>>>((uint32_t*)dst, nw); });
+}
+
+// ---------------------------------------------------------------------------
+// exclusive scan (int32 counts -> int64 offsets), three-phase
+// ---------------------------------------------------------------------------
+constexpr int SCAN_BLOCK = 1024;
+
+__global__ void k_scan_block_sums(const int32_t* __restrict__ c, int64_t n, int64_t* __restrict__ bsum) {
+  __shared__ int64_t s[SCAN_BLOCK / 64];
+  int64_t i = (int64_t)blockIdx.x * SCAN_BLOCK + threadIdx.x;
+  int64_t v = (i < n) ? c[i] : 0;
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t t = 0;
+    for (int w = 0; w < SCAN_BLOCK / 64; w++) t += s[w];
+    bsum[blockIdx.x] = t;
+  }
+}
+
+// single-block exclusive scan of the block sums (nb <= 1024*64 handled by a loop)
+__global__ void k_scan_sums(int64_t* __restrict__ bsum, int64_t nb, int64_t* __restrict__ total) {
+  __shared__ int64_t carry;
+  __shared__ int64_t ws[SCAN_BLOCK / 64];
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int64_t base = 0; base < nb; base += SCAN_BLOCK) {
+    int64_t i = base + threadIdx.x;
+    int64_t v = (i < nb) ? bsum[i] : 0;
+    // inclusive wave scan
+    int lane = threadIdx.x & 63;
+    int64_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+      int64_t y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) ws[threadIdx.x >> 6] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int64_t t = 0;
+      for (int w = 0; w < SCAN_BLOCK / 64; w++) {
+        int64_t q = ws[w];
+        ws[w] = t;
+        t += q;
+      }
+    }
+    __syncthreads();
+    int64_t excl = carry + ws[threadIdx.x >> 6] + x - v;
+    if (i < nb) bsum[i] = excl;
+    __syncthreads();
+    if (threadIdx.x == SCAN_BLOCK - 1) carry = excl + v;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ void k_scan_final(const int32_t* __restrict__ c, int64_t n, const int64_t* __restrict__ bsum,
+                             const int64_t* __restrict__ total, int64_t* __restrict__ off) {
+  __shared__ int64_t ws[SCAN_BLOCK / 64];
+  int64_t i = (int64_t)blockIdx.x * SCAN_BLOCK + threadIdx.x;
+  int64_t v = (i < n) ? c[i] : 0;
+  int lane = threadIdx.x & 63;
+  int64_t x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    int64_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) ws[threadIdx.x >> 6] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int64_t t = 0;
+    for (int w = 0; w < SCAN_BLOCK / 64; w++) {
+      int64_t q = ws[w];
+      ws[w] = t;
+      t += q;
+    }
+  }
+  __syncthreads();
+  if (i < n) off[i] = bsum[blockIdx.x] + ws[threadIdx.x >> 6] + x - v;
+  if (i == 0) off[n] = *total;
+}
+
+int64_t exclusive_scan(ipg_ctx* ctx, const int32_t* d_counts, int64_t n, int64_t* d_offsets, bool want_total) {
+  int64_t nb = (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
+  if (nb < 1) nb = 1;
+  int64_t* bsum = scratch_t<int64_t>(ctx, "scan_bsum", nb + 2);
+  int64_t* total = bsum + nb;
+  launch(ctx, "scan", [&] {
+    k_scan_block_sums<<<(unsigned)nb, SCAN_BLOCK, 0, ctx->stream>>>(d_counts, n, bsum);
+    k_scan_sums<<<1, SCAN_BLOCK, 0, ctx->stream>>>(bsum, nb, total);
+    k_scan_final<<<(unsigned)nb, SCAN_BLOCK, 0, ctx->stream>>>(d_counts, n, bsum, total, d_offsets);
+  });
+  int64_t t = 0;
+  if (want_total) {
+    d2h(ctx, &t, total, sizeof(int64_t));
+    sync(ctx);
+  }
+  return t;
+}
+
+// ---------------------------------------------------------------------------
+// deterministic row reduction: out[w] (+)= sum_i rows[i*width + w]
+// Two phases with fixed partition and fixed tree order.
+// ---------------------------------------------------------------------------
+constexpr int RED_BLOCK = 256;
+constexpr int RED_MAXW = 64;
+
+__global__ void k_reduce_rows_p1(const double* __restrict__ rows, int64_t n, int width, int64_t chunk,
+                                 double* __restrict__ part) {
+  __shared__ double sh[RED_BLOCK];
+  int64_t lo = (int64_t)blockIdx.x * chunk;
+  int64_t hi = lo + chunk < n ? lo + chunk : n;
+  for (int w = 0; w < width; w++) {
+    double acc = 0.0;
+    for (int64_t i = lo + threadIdx.x; i < hi; i += RED_BLOCK) acc += rows[i * width + w];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = RED_BLOCK / 2; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(int64_t)blockIdx.x * width + w] = sh[0];
+    __syncthreads();
+  }
+}
+
+__global__ void k_reduce_rows_p2(const double* __restrict__ part, int64_t nb, int width, double* __restrict__ out,
+                                 int accumulate) {
+  __shared__ double sh[RED_BLOCK];
+  for (int w = 0; w < width; w++) {
+    double acc = 0.0;
+    for (int64_t i = threadIdx.x; i < nb; i += RED_BLOCK) acc += part[i * width + w];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = RED_BLOCK / 2; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[w] = accumulate ? out[w] + sh[0] : sh[0];
+    __syncthreads();
+  }
+}
+
+void reduce_rows(ipg_ctx* ctx, const double* d_rows, int64_t n, int width, double* d_out, bool accumulate) {
+  int64_t nb = (n + 4095) / 4096;
+  if (nb < 1) nb = 1;
+  if (nb > 1024) nb = 1024;
+  int64_t chunk = (n + nb - 1) / nb;
+  if (chunk < 1) chunk = 1;
+  double* part = scratch_t<double>(ctx, "reduce_part", nb * (int64_t)width);
+  launch(ctx, "reduce_rows", [&] {
+    k_reduce_rows_p1<<<(unsigned)nb, RED_BLOCK, 0, ctx->stream>>>(d_rows, n, width, chunk, part);
+    k_reduce_rows_p2<<<1, RED_BLOCK, 0, ctx->stream>>>(part, nb, width, d_out, accumulate ? 1 : 0);
+  });
+}
+
+void check_system_shape(const ipg_system* s) {
+  IPG_REQUIRE(s != nullptr, "system is NULL");
+  IPG_REQUIRE(s->natoms >= 0, "natoms must be >= 0");
+  IPG_REQUIRE(s->natoms < (int64_t)INT32_MAX, "natoms must be < 2^31");
+  IPG_REQUIRE(s->natoms == 0 || s->positions != nullptr, "positions is NULL");
+  for (int d = 0; d < 3; d++)
+    if (s->periodic[d])
+      IPG_REQUIRE(std::isfinite(s->box[d]) && s->box[d] > 0.0,
+                  "box length of a periodic dimension must be finite and > 0");
+}
+
+void check_system_host(const ipg_system* s) {
+  check_system_shape(s);
+  for (int64_t i = 0; i < 3 * s->natoms; i++)
+    IPG_REQUIRE(std::isfinite(s->positions[i]), "positions must be finite (atom " + std::to_string(i / 3) + ")");
+  if (s->species)
+    for (int64_t i = 0; i < s->natoms; i++)
+      IPG_REQUIRE(s->species[i] >= 0 && s->species[i] < 64, "species index out of range [0, 64)");
+}
+
+double sqrt_le_threshold(double r) {
+  double s = r * r;
+  while (s > 0 && std::sqrt(s) > r) s = std::nextafter(s, 0.0);
+  for (;;) {
+    double t = std::nextafter(s, DBL_MAX);
+    if (std::sqrt(t) <= r) s = t;
+    else break;
+  }
+  return s;
+}
+
+}  // namespace ipg
+
+using namespace ipg;
+
+extern "C" {
+
+int ipg_get_version(void) { return IPG_VERSION; }
+
+const char* ipg_last_error(void) { return g_last_error.c_str(); }
+
+int ipg_ctx_create(int device, ipg_ctx** out) {
+  return guard([&] {
+    IPG_REQUIRE(out != nullptr, "out is NULL");
+    int ndev = 0;
+    IPG_HIP(hipGetDeviceCount(&ndev));
+    IPG_REQUIRE(device >= 0 && device < ndev, "device index out of range");
+    IPG_HIP(hipSetDevice(device));
+    auto* c = new ipg_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+      delete c;
+      fail(IPG_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
+    *out = c;
+  });
+}
+
+int ipg_ctx_destroy(ipg_ctx* ctx) {
+  return guard([&] {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& kv : ctx->scratch)
+      if (kv.second.ptr) (void)hipFree(kv.second.ptr);
+    for (auto& p : ctx->pending) {
+      (void)hipEventDestroy(p.a);
+      (void)hipEventDestroy(p.b);
+    }
+    for (auto e : ctx->free_events) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+  });
+}
+
+int ipg_ctx_synchronize(ipg_ctx* ctx) {
+  return guard([&] {
+    IPG_REQUIRE(ctx, "ctx is NULL");
+    sync(ctx);
+  });
+}
+
+int ipg_device_malloc(ipg_ctx* ctx, size_t bytes, void** dptr) {
+  return guard([&] {
+    IPG_REQUIRE(ctx && dptr, "NULL argument");
+    IPG_HIP(hipSetDevice(ctx->device));
+    if (hipMalloc(dptr, bytes ? bytes : 1) != hipSuccess) {
+      (void)hipGetLastError();
+      fail(IPG_ERR_NOMEM, "hipMalloc failed");
+    }
+  });
+}
+
+int ipg_device_free(ipg_ctx* ctx, void* dptr) {
+  return guard([&] {
+    IPG_REQUIRE(ctx, "ctx is NULL");
+    sync(ctx);
+    if (dptr) IPG_HIP(hipFree(dptr));
+  });
+}
+
+int ipg_memcpy_h2d(ipg_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  return guard([&] {
+    IPG_REQUIRE(ctx, "ctx is NULL");
+    h2d(ctx, dst, src, bytes);
+    sync(ctx);
+  });
+}
+
+int ipg_memcpy_d2h(ipg_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  return guard([&] {
+    IPG_REQUIRE(ctx, "ctx is NULL");
+    d2h(ctx, dst, src, bytes);
+    sync(ctx);
+  });
+}
+
+int ipg_prof_enable(ipg_ctx* ctx, int on) {
+  return guard([&] {
+    IPG_REQUIRE(ctx, "ctx is NULL");
+    flush_prof(ctx);
+    ctx->prof = on != 0;
+  });
+}
+
+int ipg_prof_reset(ipg_ctx* ctx) {
+  return guard([&] {
+    IPG_REQUIRE(ctx, "ctx is NULL");
+    flush_prof(ctx);
+    ctx->stats.clear();
+  });
+}
+
+int ipg_prof_query(ipg_ctx* ctx, const char* kernel, int64_t* launches, double* total_ms) {
+  return guard([&] {
+    IPG_REQUIRE(ctx && kernel, "NULL argument");
+    flush_prof(ctx);
+    auto it = ctx->stats.find(kernel);
+    if (launches) *launches = it == ctx->stats.end() ? 0 : it->second.first;
+    if (total_ms) *total_ms = it == ctx->stats.end() ? 0.0 : it->second.second;
+  });
+}
+
+}  // extern "C"

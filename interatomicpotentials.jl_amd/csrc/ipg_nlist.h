@@ -1,0 +1,67 @@
+// ipg_nlist.h -- device neighbor lists shared by the pair and SNAP paths.
+#pragma once
+#include "ipg_internal.h"
+
+namespace ipg {
+
+enum ListKind { KIND_HALF = 0, KIND_FULL = 1, KIND_FULL_SELF = 2 };
+
+// Device-resident system (pointers are device pointers).
+struct DevSystem {
+  int64_t n = 0;
+  const double* pos = nullptr;
+  const int32_t* species = nullptr;  // may be null (all species 0)
+  IpgBox box{};
+};
+
+// A CSR neighbor list on the device.
+struct DevList {
+  int64_t n = 0, npairs = 0;
+  int64_t* off = nullptr;  // n+1
+  int32_t* nbr = nullptr;  // npairs
+  double* disp = nullptr;  // 3*npairs or null
+};
+
+DevSystem make_dev_system(const ipg_system* s);  // s holds device pointers
+DevSystem upload_system(ipg_ctx* ctx, const ipg_system* s, const char* tag);  // s holds host pointers
+
+// Builds a list with membership PeriodicEuclidean distance <= rcut.
+// Storage comes from context scratch buffers named after `tag`.  sorted:
+// j ascending per atom.  want_disp: store get_displacement(x_i, x_j).
+DevList build_list(ipg_ctx* ctx, const DevSystem& s, double rcut, int kind, bool sorted, bool want_disp,
+                   const char* tag);
+
+// Same, into caller-owned device buffers allocated by the function (for
+// ipg_nlist objects); frees nothing.
+DevList build_list_owned(ipg_ctx* ctx, const DevSystem& s, double rcut, int kind, bool sorted, bool want_disp);
+
+// SNAP list of utl.jl:38-101.  rvec = x_j - x_i (min image), R = |rvec|.
+struct SnapListDev {
+  DevList list;       // nbr/off of the SNAP list
+  double* R = nullptr;
+  double* rvec = nullptr;
+  int32_t* center = nullptr;   // center atom of each pair
+  int64_t* roff = nullptr;     // reverse list offsets (n+1): pairs whose neighbor is the atom
+  int32_t* rpair = nullptr;    // reverse list pair indices, ascending
+};
+struct SnapCut {
+  int ns;
+  double cutsq[64];  // ns x ns (ns <= 8)
+  double query;      // inrange radius (sqrt(cutmax) in REFERENCE mode, cutmax otherwise)
+  int reference;     // 1: element lookup as utl.jl:70,86 (appear[] of atom and of rank m)
+};
+SnapListDev build_snap_list(ipg_ctx* ctx, const DevSystem& s, const int32_t* d_appear, const SnapCut& cut,
+                            bool want_reverse, const char* tag);
+
+// center[q] and the reverse list of any CSR list.
+void build_center_and_reverse(ipg_ctx* ctx, int64_t n, int64_t npairs, const int64_t* off, const int32_t* nbr,
+                              int32_t* center, int64_t* roff, int32_t* rpair, const char* tag);
+
+}  // namespace ipg
+
+struct ipg_nlist {
+  ipg_ctx* ctx = nullptr;
+  ipg::DevList list;
+  double rcut = 0;
+  int kind = 0;
+};

@@ -1,0 +1,366 @@
+// ipg_pair.hip -- fused pair energy / force / virial kernel.
+//
+// Semantics of src/types/empirical_potential.jl:20-67 and
+// src/mixed/linear_combination_potential.jl:27-39, fused: one full list at
+// the largest rcut; for every pair, each term whose species filter holds
+// for both atoms (empirical_potential.jl:36-38) and whose own rcut admits
+// the pair (PeriodicEuclidean distance <= rcut_k, the membership of that
+// term's own list) contributes coeff_k * V_k and coeff_k * F_k.
+//
+// Layout: LANES lanes cooperate on one atom and stride over its neighbor
+// segment (coalesced index reads); forces are gathered per atom (no
+// atomics, deterministic).  For the pair {lo, hi} (lo < hi) both atoms
+// evaluate r = get_displacement(x_lo, x_hi) and F(r) bit-identically; lo
+// adds +F and hi adds -F, exactly the reference's f[i] += fo, f[j] -= fo.
+// Energy and virial are taken from the lo side only (half-pair sums) and
+// reduced in a fixed tree.
+#include "ipg_nlist.h"
+
+namespace ipg {
+
+constexpr int MAX_TERMS = 8;
+
+struct PairTermDev {
+  int kind;
+  int check;  // 1: the term's rcut is below the list's, test membership
+  uint64_t mask;
+  double coeff;
+  double smax;  // sqrt-free membership threshold of the term's rcut
+  double q[6];  // precomputed parameters, see prepare_terms()
+};
+struct PairTable {
+  int nterms;
+  int any_check;
+  PairTermDev t[MAX_TERMS];
+};
+
+// Pair functions written from the closed forms of src/empirical/*.jl.
+// Returns V and sets fs so that F = fs * r.
+template <typename T>
+__device__ __forceinline__ T pair_eval(const PairTermDev& t, T R, T& fs) {
+  switch (t.kind) {
+    case IPG_PAIR_LJ: {  // lj.jl:35-36; q = {4 eps, 24 eps, sigma}
+      T s = (T)t.q[2] / R;
+      T s2 = s * s;
+      T s6 = s2 * s2 * s2;
+      T s12 = s6 * s6;
+      fs = (T)t.q[1] * ((T)2 * s12 - s6) / (R * R);
+      return (T)t.q[0] * (s12 - s6);
+    }
+    case IPG_PAIR_MORSE: {  // morse.jl:36-39 (force sign as in the reference); q = {D, alpha, sigma}
+      T E = exp(-(T)t.q[1] * (R - (T)t.q[2]));
+      T om = (T)1 - E;
+      fs = (T)2 * (T)t.q[0] * (T)t.q[1] * om * E / R;
+      return (T)t.q[0] * (om * om);
+    }
+    case IPG_PAIR_ZBL: {  // zbl.jl:27-36; q = {kZZe^2, 1/a_screen, ln(e)}
+      const T c[4] = {(T)0.18175, (T)0.50986, (T)0.28022, (T)0.02817};
+      const T x[4] = {(T)3.19980, (T)0.94229, (T)0.40290, (T)0.20162};
+      T d = R * (T)t.q[1];
+      T le = (T)t.q[2];
+      T phi = 0, dphi = 0;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        T ek = exp(-x[k] * d * le);  // e^(-x_k d)
+        phi += c[k] * ek;
+        dphi += x[k] * c[k] * ek;
+      }
+      dphi = -dphi;
+      T cf = (T)t.q[0];
+      fs = cf * (phi + d * dphi) / (R * R * R);
+      return cf * phi / R;
+    }
+    case IPG_PAIR_BORN_MAYER: {  // bm.jl:38-39; q = {A, rho, sigma, C, D}
+      T ex = (T)t.q[0] * exp(((T)t.q[2] - R) / (T)t.q[1]);
+      T c6 = (T)t.q[3] / R, d8 = (T)t.q[4] / R;
+      c6 = c6 * c6 * c6;
+      c6 = c6 * c6;
+      T d2 = d8 * d8;
+      T d4 = d2 * d2;
+      d8 = d4 * d4;
+      fs = ex / ((T)t.q[1] * R) + (c6 - d8) / (R * R);
+      return ex + c6 - d8;
+    }
+    case IPG_PAIR_COULOMB: {  // coulomb.jl:36-37; q = {k q1 q2}
+      fs = (T)t.q[0] / (R * R * R);
+      return (T)t.q[0] / R;
+    }
+    case IPG_PAIR_MOCK: {  // test/mocks.jl:28-33; q = {seed}
+      T v = R * R * (T)t.q[0];
+      fs = v;
+      return v;
+    }
+  }
+  fs = 0;
+  return 0;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_sum_lanes(T v, int lanes) {
+  for (int o = 1; o < lanes; o <<= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// One block: 256 threads; LANES lanes per atom.
+template <typename T, int LANES>
+__global__ __launch_bounds__(256) void k_pair(int64_t n, const int64_t* __restrict__ off,
+                                              const int32_t* __restrict__ nbr, const double* __restrict__ pos,
+                                              const int32_t* __restrict__ species, IpgBox box, PairTable tab,
+                                              double* __restrict__ forces, double* __restrict__ part) {
+  const int tid = threadIdx.x;
+  const int64_t a = ((int64_t)blockIdx.x * 256 + tid) / LANES;
+  const int lane = tid % LANES;
+  double e = 0.0, v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0, v4 = 0.0, v5 = 0.0;
+  T fx = 0, fy = 0, fz = 0;
+  if (a < n) {
+    const double xa = pos[3 * a], ya = pos[3 * a + 1], za = pos[3 * a + 2];
+    const int sa = species ? species[a] : 0;
+    const int64_t q1 = off[a + 1];
+    for (int64_t q = off[a] + lane; q < q1; q += LANES) {
+      const int32_t j = nbr[q];
+      const double xj = pos[3 * j], yj = pos[3 * j + 1], zj = pos[3 * j + 2];
+      const int sj = species ? species[j] : 0;
+      const bool lo = a < j;
+      // r = get_displacement(x_lo, x_hi)
+      double rx, ry, rz;
+      if (lo) {
+        rx = ipg_disp1(xa, xj, box.L[0], box.per[0]);
+        ry = ipg_disp1(ya, yj, box.L[1], box.per[1]);
+        rz = ipg_disp1(za, zj, box.L[2], box.per[2]);
+      } else {
+        rx = ipg_disp1(xj, xa, box.L[0], box.per[0]);
+        ry = ipg_disp1(yj, ya, box.L[1], box.per[1]);
+        rz = ipg_disp1(zj, za, box.L[2], box.per[2]);
+      }
+      double spe = 0.0;
+      if (tab.any_check) spe = ipg_pe_sumsq(box, xa, ya, za, xj, yj, zj);
+      const T trx = (T)rx, try_ = (T)ry, trz = (T)rz;
+      const T R = sqrt(trx * trx + try_ * try_ + trz * trz);
+      T vsum = 0, fsum = 0;
+      for (int k = 0; k < tab.nterms; k++) {
+        const PairTermDev& t = tab.t[k];
+        if (!((t.mask >> sa) & 1ull) || !((t.mask >> sj) & 1ull)) continue;
+        if (t.check && !(spe <= t.smax)) continue;
+        T fs;
+        T vv = pair_eval<T>(t, R, fs);
+        vsum += (T)t.coeff * vv;
+        fsum += (T)t.coeff * fs;
+      }
+      const T sgn = lo ? (T)1 : (T)-1;
+      fx += sgn * fsum * trx;
+      fy += sgn * fsum * try_;
+      fz += sgn * fsum * trz;
+      if (lo) {
+        const double fd = (double)fsum;
+        e += (double)vsum;
+        const double Fx = fd * rx, Fy = fd * ry, Fz = fd * rz;
+        v0 += rx * Fx;
+        v1 += ry * Fy;
+        v2 += rz * Fz;
+        v3 += rz * Fy;
+        v4 += rz * Fx;
+        v5 += ry * Fx;
+      }
+    }
+  }
+  // per-atom force: sum over the LANES lanes (in double)
+  double dfx = wave_sum_lanes<double>((double)fx, LANES);
+  double dfy = wave_sum_lanes<double>((double)fy, LANES);
+  double dfz = wave_sum_lanes<double>((double)fz, LANES);
+  if (a < n && lane == 0 && forces) {
+    forces[3 * a] = dfx;
+    forces[3 * a + 1] = dfy;
+    forces[3 * a + 2] = dfz;
+  }
+  // block partial of energy and virial (fixed order)
+  __shared__ double sh[7][4];
+  double vals[7] = {e, v0, v1, v2, v3, v4, v5};
+#pragma unroll
+  for (int w = 0; w < 7; w++) {
+    double x = vals[w];
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    if ((tid & 63) == 0) sh[w][tid >> 6] = x;
+  }
+  __syncthreads();
+  if (tid < 7) part[(int64_t)blockIdx.x * 7 + tid] = ((sh[tid][0] + sh[tid][1]) + sh[tid][2]) + sh[tid][3];
+}
+
+__global__ void k_split_ev(const double* __restrict__ ev, double* __restrict__ energy, double* __restrict__ virial) {
+  int t = threadIdx.x;
+  if (t == 0 && energy) energy[0] = ev[0];
+  if (t >= 1 && t < 7 && virial) virial[t - 1] = ev[t];
+}
+
+static PairTable prepare_terms(const ipg_pair_term* terms, int nterms, double list_rcut) {
+  IPG_REQUIRE(nterms >= 1 && nterms <= MAX_TERMS, "nterms must be in [1, 8]");
+  PairTable tab{};
+  tab.nterms = nterms;
+  double list_smax = sqrt_le_threshold(list_rcut);
+  for (int k = 0; k < nterms; k++) {
+    const ipg_pair_term& s = terms[k];
+    PairTermDev& t = tab.t[k];
+    IPG_REQUIRE(std::isfinite(s.rcut) && s.rcut > 0, "term rcutoff must be finite and > 0");
+    IPG_REQUIRE(std::isfinite(s.coeff), "term coefficient must be finite");
+    t.kind = s.kind;
+    t.coeff = s.coeff;
+    t.mask = s.species_mask;
+    t.smax = sqrt_le_threshold(s.rcut);
+    t.check = t.smax < list_smax ? 1 : 0;
+    tab.any_check |= t.check;
+    const double* p = s.p;
+    switch (s.kind) {
+      case IPG_PAIR_LJ:
+        t.q[0] = 4.0 * p[0];
+        t.q[1] = 24.0 * p[0];
+        t.q[2] = p[1];
+        break;
+      case IPG_PAIR_MORSE:
+        t.q[0] = p[0];
+        t.q[1] = p[1];
+        t.q[2] = p[2];
+        break;
+      case IPG_PAIR_ZBL: {
+        IPG_REQUIRE(p[2] > 0, "ZBL charge parameter e must be > 0");
+        t.q[0] = 1.0 * p[0] * p[1] * (p[2] * p[2]);
+        t.q[1] = (std::pow(p[0], 0.23) + std::pow(p[1], 0.23)) / (0.8854 * 0.529);
+        t.q[2] = std::log(p[2]);
+        break;
+      }
+      case IPG_PAIR_BORN_MAYER:
+        for (int i = 0; i < 5; i++) t.q[i] = p[i];
+        break;
+      case IPG_PAIR_COULOMB:
+        t.q[0] = 1.0 * p[0] * p[1];
+        break;
+      case IPG_PAIR_MOCK:
+        t.q[0] = p[0];
+        break;
+      default:
+        fail(IPG_ERR_INVALID, "unknown pair term kind " + std::to_string(s.kind));
+    }
+  }
+  return tab;
+}
+
+static double max_rcut(const ipg_pair_term* terms, int nterms) {
+  IPG_REQUIRE(terms != nullptr, "terms is NULL");
+  IPG_REQUIRE(nterms >= 1 && nterms <= MAX_TERMS, "nterms must be in [1, 8]");
+  double r = 0;
+  for (int k = 0; k < nterms; k++) {
+    IPG_REQUIRE(std::isfinite(terms[k].rcut) && terms[k].rcut > 0, "term rcutoff must be finite and > 0");
+    r = std::max(r, terms[k].rcut);
+  }
+  return r;
+}
+
+static int pick_lanes(int64_t n, int64_t npairs) {
+  double avg = n > 0 ? (double)npairs / (double)n : 0.0;
+  if (avg >= 48) return 16;
+  if (avg >= 12) return 8;
+  return 4;
+}
+
+// Launch the pair kernel over a full list; e/v to d_ev[7] (device).
+static void run_pair(ipg_ctx* ctx, const DevSystem& s, const DevList& L, const PairTable& tab, int precision,
+                     double* d_forces, double* d_ev) {
+  int lanes = pick_lanes(s.n, L.npairs);
+  int64_t nthreads = s.n * lanes;
+  int64_t nblk = (nthreads + 255) / 256;
+  if (nblk < 1) nblk = 1;
+  double* part = scratch_t<double>(ctx, "pair_part", nblk * 7);
+  auto go = [&](auto tag, auto lanes_c) {
+    using T = decltype(tag);
+    constexpr int LN = decltype(lanes_c)::value;
+    const char* nm = std::is_same<T, double>::value ? "pair_fp64" : "pair_fp32";
+    launch(ctx, nm, [&] {
+      k_pair<T, LN><<<(unsigned)nblk, 256, 0, ctx->stream>>>(s.n, L.off, L.nbr, s.pos, s.species, s.box, tab,
+                                                             d_forces, part);
+    });
+  };
+  if (precision == IPG_FP64) {
+    if (lanes == 16) go(0.0, std::integral_constant<int, 16>());
+    else if (lanes == 8) go(0.0, std::integral_constant<int, 8>());
+    else go(0.0, std::integral_constant<int, 4>());
+  } else {
+    if (lanes == 16) go(0.0f, std::integral_constant<int, 16>());
+    else if (lanes == 8) go(0.0f, std::integral_constant<int, 8>());
+    else go(0.0f, std::integral_constant<int, 4>());
+  }
+  reduce_rows(ctx, part, nblk, 7, d_ev, false);
+}
+
+static void pair_dev(ipg_ctx* ctx, const DevSystem& s, const ipg_pair_term* terms, int nterms, int precision,
+                     double* d_energy, double* d_forces, double* d_virial, const DevList* prebuilt,
+                     double prebuilt_rcut) {
+  IPG_REQUIRE(precision == IPG_FP64 || precision == IPG_FP32, "precision must be IPG_FP64 or IPG_FP32");
+  double rc = max_rcut(terms, nterms);
+  DevList L;
+  double list_rcut = prebuilt ? prebuilt_rcut : rc;
+  if (prebuilt) {
+    L = *prebuilt;
+    IPG_REQUIRE(L.n == s.n, "list natoms does not match the system");
+  } else {
+    L = build_list(ctx, s, rc, KIND_FULL, false, false, "pair_list");
+  }
+  // terms whose rcut is below the list's test their own membership
+  PairTable tab = prepare_terms(terms, nterms, list_rcut);
+  double* ev = scratch_t<double>(ctx, "pair_ev", 8);
+  double* f = d_forces ? d_forces : scratch_t<double>(ctx, "pair_fscratch", 3 * (s.n > 0 ? s.n : 1));
+  if (s.n == 0) {
+    zero(ctx, ev, sizeof(double) * 8);
+  } else {
+    run_pair(ctx, s, L, tab, precision, f, ev);
+  }
+  launch(ctx, "split_ev", [&] { k_split_ev<<<1, 64, 0, ctx->stream>>>(ev, d_energy, d_virial); });
+}
+
+}  // namespace ipg
+
+using namespace ipg;
+
+extern "C" {
+
+int ipg_pair_energy_forces(ipg_ctx* ctx, const ipg_system* sys, const ipg_pair_term* terms, int nterms,
+                           int precision, double* energy, double* forces, double* virial) {
+  return guard([&] {
+    IPG_REQUIRE(ctx && sys, "NULL argument");
+    IPG_HIP(hipSetDevice(ctx->device));
+    DevSystem s = upload_system(ctx, sys, "pair_api");
+    double* dout = scratch_t<double>(ctx, "pair_api_out", 8 + 3 * (s.n > 0 ? s.n : 1));
+    double* dev_e = dout;
+    double* dev_v = dout + 1;
+    double* dev_f = dout + 8;
+    pair_dev(ctx, s, terms, nterms, precision, dev_e, forces ? dev_f : nullptr, dev_v, nullptr, 0.0);
+    double ev[8];
+    d2h(ctx, ev, dout, sizeof(double) * 8);
+    if (forces) d2h(ctx, forces, dev_f, sizeof(double) * 3 * s.n);
+    sync(ctx);
+    if (energy) *energy = ev[0];
+    if (virial)
+      for (int k = 0; k < 6; k++) virial[k] = ev[1 + k];
+  });
+}
+
+int ipg_pair_energy_forces_dev(ipg_ctx* ctx, const ipg_system* sys, const ipg_pair_term* terms, int nterms,
+                               int precision, double* energy, double* forces, double* virial) {
+  return guard([&] {
+    IPG_REQUIRE(ctx && sys, "NULL argument");
+    IPG_HIP(hipSetDevice(ctx->device));
+    DevSystem s = make_dev_system(sys);
+    pair_dev(ctx, s, terms, nterms, precision, energy, forces, virial, nullptr, 0.0);
+  });
+}
+
+int ipg_pair_eval_dev(ipg_ctx* ctx, const ipg_nlist* full_list, const ipg_system* sys, const ipg_pair_term* terms,
+                      int nterms, int precision, double* energy, double* forces, double* virial) {
+  return guard([&] {
+    IPG_REQUIRE(ctx && sys && full_list, "NULL argument");
+    IPG_REQUIRE(full_list->kind == IPG_LIST_FULL, "ipg_pair_eval_dev needs an IPG_LIST_FULL list");
+    IPG_REQUIRE(max_rcut(terms, nterms) <= full_list->rcut, "list cutoff is below a term's rcut");
+    IPG_HIP(hipSetDevice(ctx->device));
+    DevSystem s = make_dev_system(sys);
+    pair_dev(ctx, s, terms, nterms, precision, energy, forces, virial, &full_list->list, full_list->rcut);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,171 @@
+// ipg_snap.h -- SNAP tables, device views and the forward-mode Dual scalar.
+#pragma once
+#include <array>
+#include <type_traits>
+
+#include "ipg_nlist.h"
+
+namespace ipg {
+
+constexpr int SNAP_JMAX = 8;  // largest twojmax with a compiled kernel set
+
+// One Z entry (types/prebuilt_arrays.jl:6-17 SNA_ZINDICES) plus the
+// Clebsch-Gordan block offset and the Z-block id.
+struct ZEnt {
+  int j1, j2, j, ma1min, ma2max, mb1min, mb2max, na, nb, jju, cgoff, blk;
+};
+// Contributions of one Z block (j1,j2,j) to Y_j: coef = sum_k beta[jjb[k]] * fac[k].
+struct YBlk {
+  int n;
+  int jjb[3];
+  double fac[3];
+};
+// The three half-sums of dB_jjb/dr (compute_b.jl:207-402): level, Z block
+// start, factor.
+struct BTerm {
+  int l[3];
+  int jjz0[3];
+  double fac[3];
+  int j;
+};
+
+struct SnapHost {
+  int J = 0, nspecies = 1;
+  int idxu_max = 0, idxz_max = 0, idxb_max = 0, idxcg_max = 0, nblk = 0, nhalf = 0;
+  int ncoeff = 0;
+  bool bnorm = false, bzero = false, switch_flag = false, wselfall = false;
+  double rcutfac = 0, rmin0 = 0, rfac0 = 0, wself = 1.0;
+  std::vector<double> radii, weight;
+  std::vector<int> idxu_block;                 // J+1
+  std::vector<int> idxz_block;                 // (J+1)^3
+  std::vector<int> idxcg_block;                // (J+1)^3
+  std::vector<int> idxb_block;                 // (J+1)^3
+  std::vector<std::array<int, 3>> idxb;        // idxb_max
+  std::vector<ZEnt> zent;                      // idxz_max
+  std::vector<YBlk> yblk;                      // nblk
+  std::vector<BTerm> bterm;                    // idxb_max
+  std::vector<double> cg;                      // idxcg_max
+  std::vector<double> bzero_j;                 // J+1
+  std::vector<int> half_jju;                   // nhalf left-half positions (mb <= j/2)
+  std::vector<int> yoff;                       // nhalf+1
+  std::vector<int> yjjz;                       // jjz list per left-half position
+};
+
+void snap_build_host(SnapHost& h, const ipg_snap_params* p);
+
+// Device copy of the tables (one allocation).
+struct SnapDev {
+  int J, nspecies, idxu_max, idxz_max, idxb_max, ncoeff, nblk, nhalf;
+  int bnorm, bzero, switch_flag;
+  double rcutfac, rmin0, rfac0, wself;
+  double radii[8], weight[8];
+  double bzero_j[SNAP_JMAX + 1];
+  int idxu_block[SNAP_JMAX + 2];
+  int hoff[SNAP_JMAX + 2];  // offsets of each level's left-half rows (mb <= j/2)
+  const ZEnt* zent;
+  const YBlk* yblk;
+  const BTerm* bterm;
+  const double* cg_d;
+  const float* cg_f;
+  const int* half_jju;
+  const int* yoff;
+  const int* yjjz;
+};
+
+// Per-call device data of one SNAP evaluation.
+struct SnapRun {
+  int64_t n = 0, npairs = 0;
+  const int64_t* off = nullptr;   // SNAP list offsets (n+1)
+  const int32_t* nbr = nullptr;   // neighbor of each pair
+  const int32_t* center = nullptr;
+  const double* rvec = nullptr;   // x_j - x_i per pair (3)
+  const int32_t* species = nullptr;
+  const int64_t* roff = nullptr;  // reverse list
+  const int32_t* rpair = nullptr;
+  const double* tang = nullptr;   // per-atom tangent direction (3n) for Dual runs
+};
+
+// ---------------------------------------------------------------------------
+// Dual numbers: value + tangent, for exact forward-mode directional
+// derivatives through the SNAP kernels.
+// ---------------------------------------------------------------------------
+// keep the float/double overloads visible next to the Dual ones below
+using ::cos;
+using ::sin;
+using ::sqrt;
+
+template <typename T>
+struct Dual {
+  T v, t;
+  Dual() = default;  // trivial, so Dual can live in __shared__ arrays
+  __host__ __device__ Dual(T a) : v(a), t(0) {}
+  __host__ __device__ Dual(T a, T b) : v(a), t(b) {}
+  template <typename U, typename = typename std::enable_if<std::is_arithmetic<U>::value && !std::is_same<U, T>::value>::type>
+  __host__ __device__ explicit Dual(U a) : v((T)a), t(0) {}
+};
+template <typename T> __device__ __forceinline__ Dual<T> operator+(Dual<T> a, Dual<T> b) { return {a.v + b.v, a.t + b.t}; }
+template <typename T> __device__ __forceinline__ Dual<T> operator-(Dual<T> a, Dual<T> b) { return {a.v - b.v, a.t - b.t}; }
+template <typename T> __device__ __forceinline__ Dual<T> operator-(Dual<T> a) { return {-a.v, -a.t}; }
+template <typename T> __device__ __forceinline__ Dual<T> operator*(Dual<T> a, Dual<T> b) { return {a.v * b.v, a.v * b.t + a.t * b.v}; }
+template <typename T> __device__ __forceinline__ Dual<T> operator*(Dual<T> a, T b) { return {a.v * b, a.t * b}; }
+template <typename T> __device__ __forceinline__ Dual<T> operator*(T b, Dual<T> a) { return {a.v * b, a.t * b}; }
+template <typename T> __device__ __forceinline__ Dual<T> operator/(Dual<T> a, Dual<T> b) {
+  T iv = (T)1 / b.v;
+  T q = a.v * iv;
+  return {q, (a.t - q * b.t) * iv};
+}
+template <typename T> __device__ __forceinline__ Dual<T> operator/(Dual<T> a, T b) { return {a.v / b, a.t / b}; }
+template <typename T> __device__ __forceinline__ Dual<T> operator/(T a, Dual<T> b) {
+  T q = a / b.v;
+  return {q, -q * b.t / b.v};
+}
+template <typename T> __device__ __forceinline__ Dual<T> operator+(Dual<T> a, T b) { return {a.v + b, a.t}; }
+template <typename T> __device__ __forceinline__ Dual<T> operator+(T b, Dual<T> a) { return {a.v + b, a.t}; }
+template <typename T> __device__ __forceinline__ Dual<T> operator-(Dual<T> a, T b) { return {a.v - b, a.t}; }
+template <typename T> __device__ __forceinline__ Dual<T> operator-(T b, Dual<T> a) { return {b - a.v, -a.t}; }
+template <typename T> __device__ __forceinline__ bool operator<=(Dual<T> a, T b) { return a.v <= b; }
+template <typename T> __device__ __forceinline__ bool operator>(Dual<T> a, T b) { return a.v > b; }
+template <typename T> __device__ __forceinline__ Dual<T>& operator+=(Dual<T>& a, Dual<T> b) { a.v += b.v; a.t += b.t; return a; }
+template <typename T> __device__ __forceinline__ Dual<T>& operator-=(Dual<T>& a, Dual<T> b) { a.v -= b.v; a.t -= b.t; return a; }
+template <typename T> __device__ __forceinline__ Dual<T>& operator*=(Dual<T>& a, Dual<T> b) { a = a * b; return a; }
+template <typename T> __device__ __forceinline__ bool operator<=(Dual<T> a, Dual<T> b) { return a.v <= b.v; }
+template <typename T> __device__ __forceinline__ bool operator>(Dual<T> a, Dual<T> b) { return a.v > b.v; }
+template <typename T> __device__ __forceinline__ Dual<T> sqrt(Dual<T> a) {
+  T s = sqrt(a.v);
+  return {s, a.t / ((T)2 * s)};
+}
+template <typename T> __device__ __forceinline__ Dual<T> sin(Dual<T> a) { return {sin(a.v), cos(a.v) * a.t}; }
+template <typename T> __device__ __forceinline__ Dual<T> cos(Dual<T> a) { return {cos(a.v), -sin(a.v) * a.t}; }
+
+template <typename S> struct BaseOf { using type = S; };
+template <typename T> struct BaseOf<Dual<T>> { using type = T; };
+template <typename S> __device__ __forceinline__ double value_of(S x) { return (double)x; }
+template <typename T> __device__ __forceinline__ double value_of(Dual<T> x) { return (double)x.v; }
+template <typename S> __device__ __forceinline__ double tangent_of(S) { return 0.0; }
+template <typename T> __device__ __forceinline__ double tangent_of(Dual<T> x) { return (double)x.t; }
+template <typename S> struct IsDual { static constexpr bool value = false; };
+template <typename T> struct IsDual<Dual<T>> { static constexpr bool value = true; };
+
+// Kernel launchers, instantiated per scalar type S in ipg_snap_k_*.hip.
+template <typename S>
+void snap_launch_ui(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, S* ulisttot);
+template <typename S>
+void snap_launch_zby(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const S* ulisttot, const double* beta,
+                     const typename BaseOf<S>::type* ycoef, double* Bout, double* eatom, S* Yout, S* Zout,
+                     double* tanB);
+template <typename S>
+void snap_launch_dedr(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const S* Y, double* dedr, double* dedr_t);
+template <typename T>
+void snap_launch_ycoef(ipg_ctx* ctx, const SnapDev& P, const double* beta, T* ycoef);
+void snap_launch_dbdr_plain(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const double* Z, double* dblist);
+
+}  // namespace ipg
+
+struct ipg_snap {
+  ipg_ctx* ctx = nullptr;
+  ipg::SnapHost host;
+  ipg::SnapDev dev{};
+  void* dmem = nullptr;  // tables
+  double* beta_dev = nullptr;  // expanded beta (ncoeff * nspecies), owned
+  int64_t beta_cap = 0;
+};

@@ -1,0 +1,725 @@
+// ipg_snap.hip -- SNAP tables, entry points and the deterministic gathers.
+//
+// Table semantics follow types/prebuilt_arrays.jl:48-321 (index blocks in
+// the same j1 >= j2, j = |j1-j2|..min(J, j1+j2) step 2 order; the
+// Clebsch-Gordan coefficients from the Racah closed form); the derived
+// tables (Y coefficients, dB half-sum terms) are this engine's own.
+#include "ipg_snap.h"
+
+#include <algorithm>
+
+namespace ipg {
+
+// ---------------------------------------------------------------------------
+// host tables
+// ---------------------------------------------------------------------------
+static long double lfact(int n) {
+  long double r = 1.0L;
+  for (int k = 2; k <= n; k++) r *= (long double)k;
+  return r;
+}
+
+// <j1/2 a/2, j2/2 b/2 | j/2 (a+b)/2> by the Racah formula (arguments doubled).
+static double clebsch_gordan2(int j1, int j2, int j, int a, int b) {
+  const int c = a + b;
+  if (c < -j || c > j) return 0.0;
+  const long double tri = lfact((j1 + j2 - j) / 2) * lfact((j1 - j2 + j) / 2) * lfact((-j1 + j2 + j) / 2) /
+                          lfact((j1 + j2 + j) / 2 + 1);
+  const long double nrm = (long double)(j + 1) * lfact((j1 + a) / 2) * lfact((j1 - a) / 2) * lfact((j2 + b) / 2) *
+                          lfact((j2 - b) / 2) * lfact((j + c) / 2) * lfact((j - c) / 2);
+  const int zmin = std::max(0, std::max(-(j - j2 + a) / 2, -(j - j1 - b) / 2));
+  const int zmax = std::min((j1 + j2 - j) / 2, std::min((j1 - a) / 2, (j2 + b) / 2));
+  long double s = 0.0L;
+  for (int z = zmin; z <= zmax; z++) {
+    const long double den = lfact(z) * lfact((j1 + j2 - j) / 2 - z) * lfact((j1 - a) / 2 - z) * lfact((j2 + b) / 2 - z) *
+                            lfact((j - j2 + a) / 2 + z) * lfact((j - j1 - b) / 2 + z);
+    s += ((z & 1) ? -1.0L : 1.0L) / den;
+  }
+  return (double)(std::sqrt(tri) * std::sqrt(nrm) * s);
+}
+
+void snap_build_host(SnapHost& h, const ipg_snap_params* p) {
+  IPG_REQUIRE(p != nullptr, "params is NULL");
+  IPG_REQUIRE(p->twojmax >= 0, "twojmax must be >= 0");
+  if (p->twojmax > SNAP_JMAX) fail(IPG_ERR_UNSUPPORTED, "twojmax > 8 is not compiled for the GPU");
+  IPG_REQUIRE(p->nspecies >= 1 && p->nspecies <= 8, "nspecies must be in [1, 8]");
+  IPG_REQUIRE(p->radii && p->weight, "radii/weight is NULL");
+  // snap.jl:24-27
+  IPG_REQUIRE((p->chem_flag != 0) == (p->bnorm_flag != 0),
+              "chem_flag and bnorm_flag should be the same, setting bnorm_flag == chem_flag");
+  if (p->chem_flag) fail(IPG_ERR_UNSUPPORTED, "chem_flag = true is not supported on the GPU");
+  IPG_REQUIRE(std::isfinite(p->rcutfac) && p->rcutfac > 0, "rcutfac must be finite and > 0");
+  IPG_REQUIRE(std::isfinite(p->rfac0) && std::isfinite(p->rmin0), "rfac0/rmin0 must be finite");
+  for (int s = 0; s < p->nspecies; s++) {
+    IPG_REQUIRE(std::isfinite(p->radii[s]) && p->radii[s] > 0, "radii must be finite and > 0");
+    IPG_REQUIRE(std::isfinite(p->weight[s]), "weight must be finite");
+    IPG_REQUIRE((p->radii[s] + p->radii[s]) * p->rcutfac > p->rmin0, "cutoff must exceed rmin0");
+  }
+  const int J = p->twojmax, jd = J + 1;
+  h.J = J;
+  h.nspecies = p->nspecies;
+  h.bnorm = p->bnorm_flag != 0;
+  h.bzero = p->bzero_flag != 0;
+  h.switch_flag = p->switch_flag != 0;
+  h.wselfall = p->wselfall_flag != 0;
+  h.rcutfac = p->rcutfac;
+  h.rmin0 = p->rmin0;
+  h.rfac0 = p->rfac0;
+  h.radii.assign(p->radii, p->radii + p->nspecies);
+  h.weight.assign(p->weight, p->weight + p->nspecies);
+  auto b3 = [&](int x, int y, int z) { return (x * jd + y) * jd + z; };
+  h.idxu_block.assign(jd + 1, 0);
+  for (int j = 0; j < jd; j++) h.idxu_block[j + 1] = h.idxu_block[j] + (j + 1) * (j + 1);
+  h.idxu_max = h.idxu_block[jd];
+  h.idxcg_block.assign(jd * jd * jd, -1);
+  h.idxz_block.assign(jd * jd * jd, -1);
+  h.idxb_block.assign(jd * jd * jd, -1);
+  h.idxb.clear();
+  h.zent.clear();
+  h.yblk.clear();
+  h.cg.clear();
+  std::vector<std::array<int, 3>> blocks;
+  for (int j1 = 0; j1 <= J; j1++)
+    for (int j2 = 0; j2 <= j1; j2++)
+      for (int j = j1 - j2; j <= std::min(J, j1 + j2); j += 2) {
+        h.idxcg_block[b3(j1, j2, j)] = (int)h.cg.size();
+        for (int m1 = 0; m1 <= j1; m1++)
+          for (int m2 = 0; m2 <= j2; m2++) h.cg.push_back(clebsch_gordan2(j1, j2, j, 2 * m1 - j1, 2 * m2 - j2));
+        if (j >= j1) {
+          h.idxb_block[b3(j1, j2, j)] = (int)h.idxb.size();
+          h.idxb.push_back({j1, j2, j});
+        }
+        blocks.push_back({j1, j2, j});
+      }
+  h.idxcg_max = (int)h.cg.size();
+  h.idxb_max = (int)h.idxb.size();
+  h.ncoeff = h.idxb_max;  // get_num_snap_coeffs with chem_flag = false
+  h.nblk = (int)blocks.size();
+  for (int bk = 0; bk < h.nblk; bk++) {
+    const int j1 = blocks[bk][0], j2 = blocks[bk][1], j = blocks[bk][2];
+    h.idxz_block[b3(j1, j2, j)] = (int)h.zent.size();
+    for (int mb = 0; 2 * mb <= j; mb++)
+      for (int ma = 0; ma <= j; ma++) {
+        ZEnt z;
+        z.j1 = j1;
+        z.j2 = j2;
+        z.j = j;
+        z.ma1min = std::max(0, (2 * ma - j - j2 + j1) / 2);
+        z.ma2max = (2 * ma - j - (2 * z.ma1min - j1) + j2) / 2;
+        z.na = std::min(j1, (2 * ma - j + j2 + j1) / 2) - z.ma1min + 1;
+        z.mb1min = std::max(0, (2 * mb - j - j2 + j1) / 2);
+        z.mb2max = (2 * mb - j - (2 * z.mb1min - j1) + j2) / 2;
+        z.nb = std::min(j1, (2 * mb - j + j2 + j1) / 2) - z.mb1min + 1;
+        z.jju = h.idxu_block[j] + (j + 1) * mb + ma;
+        z.cgoff = h.idxcg_block[b3(j1, j2, j)];
+        z.blk = bk;
+        h.zent.push_back(z);
+      }
+  }
+  h.idxz_max = (int)h.zent.size();
+  // Y coefficients: the (triple, role) pairs whose dB/dr half-sum uses the
+  // Z block (j1, j2, j) at level j.
+  const double one = 1.0;
+  for (int bk = 0; bk < h.nblk; bk++) {
+    const int j1 = blocks[bk][0], j2 = blocks[bk][1], j = blocks[bk][2];
+    YBlk y{};
+    y.n = 0;
+    if (j >= j1) {  // role j of (j1, j2, j)
+      y.jjb[y.n] = h.idxb_block[b3(j1, j2, j)];
+      y.fac[y.n++] = one;
+    }
+    if (j1 >= j && j >= j2) {  // role j1 of (j, j2, j1)
+      y.jjb[y.n] = h.idxb_block[b3(j, j2, j1)];
+      y.fac[y.n++] = h.bnorm ? one : (j1 + 1) / (j + 1.0);
+    }
+    if (j1 >= j2 && j2 >= j) {  // role j2 of (j2, j, j1)
+      y.jjb[y.n] = h.idxb_block[b3(j2, j, j1)];
+      y.fac[y.n++] = h.bnorm ? one : (j1 + 1) / (j + 1.0);
+    }
+    for (int k = 0; k < y.n; k++) IPG_REQUIRE(y.jjb[k] >= 0, "internal: bad Y block");
+    h.yblk.push_back(y);
+  }
+  // dB/dr half-sum terms per triple (compute_b.jl:207-399)
+  h.bterm.clear();
+  for (int jjb = 0; jjb < h.idxb_max; jjb++) {
+    const int j1 = h.idxb[jjb][0], j2 = h.idxb[jjb][1], j = h.idxb[jjb][2];
+    BTerm t;
+    t.j = j;
+    t.l[0] = j;
+    t.jjz0[0] = h.idxz_block[b3(j1, j2, j)];
+    t.fac[0] = 1.0;
+    t.l[1] = j1;
+    t.jjz0[1] = h.idxz_block[b3(j, j2, j1)];
+    t.fac[1] = h.bnorm ? 1.0 : (j + 1) / (j1 + 1.0);
+    t.l[2] = j2;
+    t.jjz0[2] = h.idxz_block[b3(j, j1, j2)];
+    t.fac[2] = h.bnorm ? 1.0 : (j + 1) / (j2 + 1.0);
+    IPG_REQUIRE(t.jjz0[0] >= 0 && t.jjz0[1] >= 0 && t.jjz0[2] >= 0, "internal: bad dB block");
+    h.bterm.push_back(t);
+  }
+  // left-half positions (mb <= j/2) and the Z entries mapping to each
+  h.half_jju.clear();
+  h.yoff.assign(1, 0);
+  h.yjjz.clear();
+  std::vector<std::vector<int>> byjju(h.idxu_max);
+  for (int k = 0; k < h.idxz_max; k++) byjju[h.zent[k].jju].push_back(k);
+  for (int j = 0; j <= J; j++)
+    for (int mb = 0; 2 * mb <= j; mb++)
+      for (int ma = 0; ma <= j; ma++) {
+        const int jju = h.idxu_block[j] + (j + 1) * mb + ma;
+        h.half_jju.push_back(jju);
+        for (int k : byjju[jju]) h.yjjz.push_back(k);
+        h.yoff.push_back((int)h.yjjz.size());
+      }
+  h.nhalf = (int)h.half_jju.size();
+  h.wself = 1.0;
+  h.bzero_j.assign(jd, 0.0);
+  if (h.bzero)
+    for (int j = 0; j <= J; j++) h.bzero_j[j] = h.bnorm ? 1.0 : (double)(j + 1);  // wself^3 = 1
+}
+
+static void snap_upload(ipg_snap* sn) {
+  const SnapHost& h = sn->host;
+  SnapDev& d = sn->dev;
+  d.J = h.J;
+  d.nspecies = h.nspecies;
+  d.idxu_max = h.idxu_max;
+  d.idxz_max = h.idxz_max;
+  d.idxb_max = h.idxb_max;
+  d.ncoeff = h.ncoeff;
+  d.nblk = h.nblk;
+  d.nhalf = h.nhalf;
+  d.bnorm = h.bnorm;
+  d.bzero = h.bzero;
+  d.switch_flag = h.switch_flag;
+  d.rcutfac = h.rcutfac;
+  d.rmin0 = h.rmin0;
+  d.rfac0 = h.rfac0;
+  d.wself = h.wself;
+  for (int s = 0; s < 8; s++) {
+    d.radii[s] = s < h.nspecies ? h.radii[s] : 0.0;
+    d.weight[s] = s < h.nspecies ? h.weight[s] : 0.0;
+  }
+  for (int j = 0; j <= SNAP_JMAX; j++) d.bzero_j[j] = j <= h.J ? h.bzero_j[j] : 0.0;
+  for (int j = 0; j <= SNAP_JMAX + 1; j++) d.idxu_block[j] = j <= h.J + 1 ? h.idxu_block[j] : 0;
+  d.hoff[0] = 0;
+  for (int j = 0; j <= SNAP_JMAX; j++) d.hoff[j + 1] = d.hoff[j] + (j / 2 + 1) * (j + 1);
+  // one allocation for every table
+  std::vector<float> cgf(h.cg.begin(), h.cg.end());
+  size_t off = 0;
+  auto place = [&](size_t bytes) {
+    size_t o = (off + 255) & ~(size_t)255;
+    off = o + bytes;
+    return o;
+  };
+  size_t o_z = place(sizeof(ZEnt) * h.zent.size());
+  size_t o_y = place(sizeof(YBlk) * h.yblk.size());
+  size_t o_b = place(sizeof(BTerm) * h.bterm.size());
+  size_t o_cd = place(sizeof(double) * h.cg.size());
+  size_t o_cf = place(sizeof(float) * cgf.size());
+  size_t o_hj = place(sizeof(int) * h.half_jju.size());
+  size_t o_yo = place(sizeof(int) * h.yoff.size());
+  size_t o_yj = place(sizeof(int) * std::max<size_t>(1, h.yjjz.size()));
+  if (hipMalloc(&sn->dmem, off) != hipSuccess) {
+    (void)hipGetLastError();
+    sn->dmem = nullptr;
+    fail(IPG_ERR_NOMEM, "hipMalloc of SNAP tables failed");
+  }
+  char* base = (char*)sn->dmem;
+  ipg_ctx* ctx = sn->ctx;
+  h2d(ctx, base + o_z, h.zent.data(), sizeof(ZEnt) * h.zent.size());
+  h2d(ctx, base + o_y, h.yblk.data(), sizeof(YBlk) * h.yblk.size());
+  h2d(ctx, base + o_b, h.bterm.data(), sizeof(BTerm) * h.bterm.size());
+  h2d(ctx, base + o_cd, h.cg.data(), sizeof(double) * h.cg.size());
+  h2d(ctx, base + o_cf, cgf.data(), sizeof(float) * cgf.size());
+  h2d(ctx, base + o_hj, h.half_jju.data(), sizeof(int) * h.half_jju.size());
+  h2d(ctx, base + o_yo, h.yoff.data(), sizeof(int) * h.yoff.size());
+  if (!h.yjjz.empty()) h2d(ctx, base + o_yj, h.yjjz.data(), sizeof(int) * h.yjjz.size());
+  sync(ctx);
+  d.zent = (const ZEnt*)(base + o_z);
+  d.yblk = (const YBlk*)(base + o_y);
+  d.bterm = (const BTerm*)(base + o_b);
+  d.cg_d = (const double*)(base + o_cd);
+  d.cg_f = (const float*)(base + o_cf);
+  d.half_jju = (const int*)(base + o_hj);
+  d.yoff = (const int*)(base + o_yo);
+  d.yjjz = (const int*)(base + o_yj);
+}
+
+// ---------------------------------------------------------------------------
+// gathers (deterministic: fixed pair order per atom, no atomics)
+// ---------------------------------------------------------------------------
+__global__ void k_force_gather(int64_t n, const int64_t* __restrict__ off, const int64_t* __restrict__ roff,
+                               const int32_t* __restrict__ rpair, const double* __restrict__ dedr,
+                               double* __restrict__ F) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double fx = 0, fy = 0, fz = 0;
+  for (int64_t q = off[i]; q < off[i + 1]; q++) {  // snap.jl:264 (forces[i] += dE_i/dr_ij)
+    fx += dedr[3 * q];
+    fy += dedr[3 * q + 1];
+    fz += dedr[3 * q + 2];
+  }
+  for (int64_t r = roff[i]; r < roff[i + 1]; r++) {  // snap.jl:265 (forces[j] -= dE_i/dr_ij)
+    const int64_t q = rpair[r];
+    fx -= dedr[3 * q];
+    fy -= dedr[3 * q + 1];
+    fz -= dedr[3 * q + 2];
+  }
+  F[3 * i] = fx;
+  F[3 * i + 1] = fy;
+  F[3 * i + 2] = fz;
+}
+
+// dB[a] (M x 3, blocks by the species of the pair's center atom).
+__global__ void k_db_gather(SnapDev P, SnapRun R, const double* __restrict__ dblist, double* __restrict__ dB) {
+  const int64_t a = blockIdx.x;
+  const int M = P.ncoeff * P.nspecies;
+  const int sa = R.species ? R.species[a] : 0;
+  for (int t = threadIdx.x; t < 3 * P.ncoeff; t += blockDim.x) {
+    const int c = t / 3, k = t % 3;
+    for (int s = 0; s < P.nspecies; s++) {
+      double acc = 0.0;
+      if (s == sa)
+        for (int64_t q = R.off[a]; q < R.off[a + 1]; q++) acc += dblist[(q * P.ncoeff + c) * 3 + k];
+      for (int64_t r = R.roff[a]; r < R.roff[a + 1]; r++) {
+        const int64_t q = R.rpair[r];
+        const int sc = R.species ? R.species[R.center[q]] : 0;
+        if (sc == s) acc -= dblist[(q * P.ncoeff + c) * 3 + k];
+      }
+      dB[(a * M + s * P.ncoeff + c) * 3 + k] = acc;
+    }
+  }
+}
+
+__global__ void k_force_from_db(int64_t n, int M, const double* __restrict__ dB, const double* __restrict__ beta,
+                                double* __restrict__ F) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * n) return;
+  const int64_t a = t / 3;
+  const int k = (int)(t % 3);
+  double f = 0.0;
+  for (int m = 0; m < M; m++) f += beta[m] * dB[(a * M + m) * 3 + k];  // linear_potential.jl:28-35
+  F[t] = f;
+}
+
+// Per-center-atom virial rows (snap.jl:267-281): sum over its pairs of
+// [dBx x, dBy y, dBz z, dBy z, dBx z, dBx y] with x = x_i - x_j (raw).
+__global__ void k_w_atom(SnapDev P, SnapRun R, const double* __restrict__ pos, const double* __restrict__ dblist,
+                         double* __restrict__ Wa) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int M = P.ncoeff * P.nspecies;
+  if (t >= R.n * (int64_t)M) return;
+  const int64_t a = t / M;
+  const int m = (int)(t % M);
+  const int sa = R.species ? R.species[a] : 0;
+  double w[6] = {0, 0, 0, 0, 0, 0};
+  if (m / P.ncoeff == sa) {
+    const int c = m % P.ncoeff;
+    for (int64_t q = R.off[a]; q < R.off[a + 1]; q++) {
+      const int32_t jn = R.nbr[q];
+      const double* db = dblist + (q * P.ncoeff + c) * 3;
+      const double xi = pos[3 * a], yi = pos[3 * a + 1], zi = pos[3 * a + 2];
+      const double xj = pos[3 * jn], yj = pos[3 * jn + 1], zj = pos[3 * jn + 2];
+      w[0] += db[0] * xi - db[0] * xj;
+      w[1] += db[1] * yi - db[1] * yj;
+      w[2] += db[2] * zi - db[2] * zj;
+      w[3] += db[1] * zi - db[1] * zj;
+      w[4] += db[0] * zi - db[0] * zj;
+      w[5] += db[0] * yi - db[0] * yj;
+    }
+  }
+  for (int v = 0; v < 6; v++) Wa[t * 6 + v] = w[v];
+}
+
+__global__ void k_finish_energy(const double* __restrict__ sum, double beta0, double* __restrict__ out) {
+  if (threadIdx.x == 0) out[0] = beta0 + sum[0];
+}
+
+// ---------------------------------------------------------------------------
+// drivers
+// ---------------------------------------------------------------------------
+struct SnapCall {
+  SnapListDev L;
+  SnapRun R;
+};
+
+static SnapCut make_cut(const SnapHost& h, int mode) {
+  SnapCut c{};
+  c.ns = h.nspecies;
+  double cutmax = 0.0;
+  for (int i = 0; i < h.nspecies; i++) {  // utl.jl:42-56
+    const double cut = 2.0 * h.radii[i] * h.rcutfac;
+    cutmax = std::max(cutmax, cut);
+    c.cutsq[i * h.nspecies + i] = cut * cut;
+    for (int j = i + 1; j < h.nspecies; j++) {
+      const double cj = (h.radii[i] + h.radii[j]) * h.rcutfac;
+      c.cutsq[i * h.nspecies + j] = cj * cj;
+      c.cutsq[j * h.nspecies + i] = cj * cj;
+    }
+  }
+  c.reference = mode == IPG_SNAP_LIST_REFERENCE ? 1 : 0;
+  c.query = c.reference ? std::sqrt(cutmax) : cutmax;  // utl.jl:81
+  return c;
+}
+
+static SnapCall snap_prepare(ipg_snap* sn, const DevSystem& s, int mode, const int32_t* d_appear, bool reverse) {
+  IPG_REQUIRE(mode == IPG_SNAP_LIST_REFERENCE || mode == IPG_SNAP_LIST_CUTOFF, "unknown SNAP list mode");
+  SnapCall c;
+  SnapCut cut = make_cut(sn->host, mode);
+  const int32_t* ap = d_appear ? d_appear : s.species;
+  if (cut.reference && !ap) {
+    // all atoms are species 0: element index 0 everywhere
+    int32_t* z = scratch_t<int32_t>(sn->ctx, "snap_appear0", s.n > 0 ? s.n : 1);
+    zero(sn->ctx, z, sizeof(int32_t) * (s.n > 0 ? s.n : 1));
+    ap = z;
+  }
+  c.L = build_snap_list(sn->ctx, s, ap, cut, true, "snap_list");
+  (void)reverse;
+  c.R.n = s.n;
+  c.R.npairs = c.L.list.npairs;
+  c.R.off = c.L.list.off;
+  c.R.nbr = c.L.list.nbr;
+  c.R.center = c.L.center;
+  c.R.rvec = c.L.rvec;
+  c.R.species = s.species;
+  c.R.roff = c.L.roff;
+  c.R.rpair = c.L.rpair;
+  return c;
+}
+
+static void check_species_host(const ipg_snap* sn, const ipg_system* sys, const int32_t* appear) {
+  for (int64_t i = 0; i < sys->natoms; i++) {
+    if (sys->species)
+      IPG_REQUIRE(sys->species[i] >= 0 && sys->species[i] < sn->host.nspecies,
+                  "species index out of range of snap.species");
+    if (appear)
+      IPG_REQUIRE(appear[i] >= 0 && appear[i] < sn->host.nspecies,
+                  "element index (first appearance order) out of range of snap.species");
+  }
+}
+
+static const int32_t* upload_appear(ipg_ctx* ctx, const int32_t* appear, int64_t n) {
+  if (!appear) return nullptr;
+  int32_t* d = scratch_t<int32_t>(ctx, "snap_appear", n > 0 ? n : 1);
+  h2d(ctx, d, appear, sizeof(int32_t) * n);
+  return d;
+}
+
+static const double* expand_beta(ipg_snap* sn, const double* beta, int64_t nbeta) {
+  const int nc = sn->host.ncoeff, ns = sn->host.nspecies;
+  IPG_REQUIRE(beta != nullptr, "beta is NULL");
+  IPG_REQUIRE(nbeta == nc || nbeta == (int64_t)nc * ns, "beta must have ncoeff or ncoeff*nspecies entries");
+  std::vector<double> b((size_t)nc * ns);
+  for (int s = 0; s < ns; s++)
+    for (int c = 0; c < nc; c++) b[(size_t)s * nc + c] = nbeta == nc ? beta[c] : beta[(size_t)s * nc + c];
+  for (double v : b) IPG_REQUIRE(std::isfinite(v), "beta must be finite");
+  double* d = scratch_t<double>(sn->ctx, "snap_beta", b.size());
+  h2d(sn->ctx, d, b.data(), sizeof(double) * b.size());
+  return d;
+}
+
+// U -> Z/B (and optionally Z to global) in FP64.  Returns the Z buffer.
+static double* run_descriptors_fp64(ipg_snap* sn, const SnapCall& c, double* dB_out, double* B_out, double* eatom,
+                                    const double* beta, bool want_z) {
+  ipg_ctx* ctx = sn->ctx;
+  const SnapDev& P = sn->dev;
+  const int64_t n = c.R.n;
+  double* U = scratch_t<double>(ctx, "snap_U64", 2 * (size_t)P.idxu_max * std::max<int64_t>(n, 1));
+  double* Z = want_z ? scratch_t<double>(ctx, "snap_Z64", 2 * (size_t)P.idxz_max * std::max<int64_t>(n, 1)) : nullptr;
+  snap_launch_ui<double>(ctx, P, c.R, U);
+  snap_launch_zby<double>(ctx, P, c.R, U, beta, nullptr, B_out, eatom, nullptr, Z, nullptr);
+  (void)dB_out;
+  return Z;
+}
+
+static double* run_dblist(ipg_snap* sn, const SnapCall& c, const double* Z) {
+  const SnapDev& P = sn->dev;
+  double* dbl = scratch_t<double>(sn->ctx, "snap_dblist", 3 * (size_t)P.ncoeff * std::max<int64_t>(c.R.npairs, 1));
+  snap_launch_dbdr_plain(sn->ctx, P, c.R, Z, dbl);
+  return dbl;
+}
+
+static void gather_db(ipg_snap* sn, const SnapCall& c, const double* dbl, double* dB) {
+  if (c.R.n == 0) return;
+  launch(sn->ctx, "snap_db_gather",
+         [&] { k_db_gather<<<(unsigned)c.R.n, 64, 0, sn->ctx->stream>>>(sn->dev, c.R, dbl, dB); });
+}
+
+// Energy and forces on the device; d_energy/d_forces are device pointers.
+static void energy_forces_dev(ipg_snap* sn, const SnapCall& c, const double* d_beta, double beta0, int path,
+                              double* d_energy, double* d_forces) {
+  ipg_ctx* ctx = sn->ctx;
+  const SnapDev& P = sn->dev;
+  const int64_t n = c.R.n;
+  const int M = P.ncoeff * P.nspecies;
+  double* eatom = scratch_t<double>(ctx, "snap_eatom", std::max<int64_t>(n, 1));
+  double* esum = scratch_t<double>(ctx, "snap_esum", 2);
+  double* F = d_forces ? d_forces : scratch_t<double>(ctx, "snap_Fscratch", 3 * std::max<int64_t>(n, 1));
+  if (n == 0) {
+    zero(ctx, esum, sizeof(double) * 2);
+  } else if (path == IPG_SNAP_PLAIN_FP64) {
+    double* Z = run_descriptors_fp64(sn, c, nullptr, nullptr, eatom, d_beta, true);
+    double* dbl = run_dblist(sn, c, Z);
+    double* dB = scratch_t<double>(ctx, "snap_dB", 3 * (size_t)M * n);
+    gather_db(sn, c, dbl, dB);
+    launch(ctx, "snap_force_from_db",
+           [&] { k_force_from_db<<<grid_for(3 * n, 256), 256, 0, ctx->stream>>>(n, M, dB, d_beta, F); });
+    reduce_rows(ctx, eatom, n, 1, esum, false);
+  } else {
+    double* dedr = scratch_t<double>(ctx, "snap_dedr", 3 * std::max<int64_t>(c.R.npairs, 1));
+    if (path == IPG_SNAP_ADJOINT_FP64) {
+      double* U = scratch_t<double>(ctx, "snap_U64", 2 * (size_t)P.idxu_max * n);
+      double* Y = scratch_t<double>(ctx, "snap_Y64", 2 * (size_t)P.idxu_max * n);
+      double* yc = scratch_t<double>(ctx, "snap_yc64", (size_t)P.nspecies * P.nblk);
+      snap_launch_ycoef<double>(ctx, P, d_beta, yc);
+      snap_launch_ui<double>(ctx, P, c.R, U);
+      snap_launch_zby<double>(ctx, P, c.R, U, d_beta, yc, nullptr, eatom, Y, nullptr, nullptr);
+      snap_launch_dedr<double>(ctx, P, c.R, Y, dedr, nullptr);
+    } else if (path == IPG_SNAP_ADJOINT_FP32) {
+      float* U = scratch_t<float>(ctx, "snap_U32", 2 * (size_t)P.idxu_max * n);
+      float* Y = scratch_t<float>(ctx, "snap_Y32", 2 * (size_t)P.idxu_max * n);
+      float* yc = scratch_t<float>(ctx, "snap_yc32", (size_t)P.nspecies * P.nblk);
+      snap_launch_ycoef<float>(ctx, P, d_beta, yc);
+      snap_launch_ui<float>(ctx, P, c.R, U);
+      snap_launch_zby<float>(ctx, P, c.R, U, d_beta, yc, nullptr, eatom, Y, nullptr, nullptr);
+      snap_launch_dedr<float>(ctx, P, c.R, Y, dedr, nullptr);
+    } else {
+      fail(IPG_ERR_INVALID, "unknown SNAP path");
+    }
+    launch(ctx, "snap_force_gather", [&] {
+      k_force_gather<<<grid_for(n, 256), 256, 0, ctx->stream>>>(n, c.R.off, c.R.roff, c.R.rpair, dedr, F);
+    });
+    reduce_rows(ctx, eatom, n, 1, esum, false);
+  }
+  if (d_energy)
+    launch(ctx, "snap_finish_energy", [&] { k_finish_energy<<<1, 64, 0, ctx->stream>>>(esum, beta0, d_energy); });
+}
+
+}  // namespace ipg
+
+using namespace ipg;
+
+extern "C" {
+
+int ipg_snap_create(ipg_ctx* ctx, const ipg_snap_params* params, ipg_snap** out) {
+  return guard([&] {
+    IPG_REQUIRE(ctx && params && out, "NULL argument");
+    IPG_HIP(hipSetDevice(ctx->device));
+    auto* sn = new ipg_snap();
+    sn->ctx = ctx;
+    try {
+      snap_build_host(sn->host, params);
+      snap_upload(sn);
+    } catch (...) {
+      if (sn->dmem) (void)hipFree(sn->dmem);
+      delete sn;
+      throw;
+    }
+    *out = sn;
+  });
+}
+
+int ipg_snap_destroy(ipg_snap* sn) {
+  return guard([&] {
+    if (!sn) return;
+    if (sn->ctx) (void)hipStreamSynchronize(sn->ctx->stream);
+    if (sn->dmem) (void)hipFree(sn->dmem);
+    if (sn->beta_dev) (void)hipFree(sn->beta_dev);
+    delete sn;
+  });
+}
+
+int ipg_snap_info(const ipg_snap* sn, int32_t* ncoeff, int32_t* idxu_max, int32_t* idxz_max, int32_t* idxb_max) {
+  return guard([&] {
+    IPG_REQUIRE(sn, "snap is NULL");
+    if (ncoeff) *ncoeff = sn->host.ncoeff;
+    if (idxu_max) *idxu_max = sn->host.idxu_max;
+    if (idxz_max) *idxz_max = sn->host.idxz_max;
+    if (idxb_max) *idxb_max = sn->host.idxb_max;
+  });
+}
+
+int ipg_snap_neighborlist(ipg_snap* sn, const ipg_system* sys, int mode, const int32_t* appear, int64_t* offsets,
+                          int32_t* nbr, double* R, double* r, int64_t cap, int64_t* npairs) {
+  return guard([&] {
+    IPG_REQUIRE(sn && sys && offsets && npairs, "NULL argument");
+    ipg_ctx* ctx = sn->ctx;
+    IPG_HIP(hipSetDevice(ctx->device));
+    check_species_host(sn, sys, appear);
+    DevSystem s = upload_system(ctx, sys, "snap_api");
+    SnapCall c = snap_prepare(sn, s, mode, upload_appear(ctx, appear, s.n), false);
+    d2h(ctx, offsets, c.L.list.off, sizeof(int64_t) * (s.n + 1));
+    *npairs = c.L.list.npairs;
+    if (c.L.list.npairs > cap) {
+      sync(ctx);
+      fail(IPG_ERR_CAPACITY, "capacity " + std::to_string(cap) + " < pairs " + std::to_string(c.L.list.npairs));
+    }
+    const int64_t np = c.L.list.npairs;
+    if (nbr) d2h(ctx, nbr, c.L.list.nbr, sizeof(int32_t) * np);
+    if (R) d2h(ctx, R, c.L.R, sizeof(double) * np);
+    if (r) d2h(ctx, r, c.L.rvec, sizeof(double) * 3 * np);
+    sync(ctx);
+  });
+}
+
+int ipg_snap_local_descriptors(ipg_snap* sn, const ipg_system* sys, int mode, const int32_t* appear, double* B) {
+  return guard([&] {
+    IPG_REQUIRE(sn && sys && B, "NULL argument");
+    ipg_ctx* ctx = sn->ctx;
+    IPG_HIP(hipSetDevice(ctx->device));
+    check_species_host(sn, sys, appear);
+    DevSystem s = upload_system(ctx, sys, "snap_api");
+    SnapCall c = snap_prepare(sn, s, mode, upload_appear(ctx, appear, s.n), false);
+    if (s.n == 0) return;
+    double* dB = scratch_t<double>(ctx, "snap_Bout", (size_t)sn->host.ncoeff * s.n);
+    run_descriptors_fp64(sn, c, nullptr, dB, nullptr, nullptr, false);
+    d2h(ctx, B, dB, sizeof(double) * sn->host.ncoeff * s.n);
+    sync(ctx);
+  });
+}
+
+int ipg_snap_all_descriptors(ipg_snap* sn, const ipg_system* sys, int mode, const int32_t* appear, double* B,
+                             double* dB, double* W) {
+  return guard([&] {
+    IPG_REQUIRE(sn && sys, "NULL argument");
+    ipg_ctx* ctx = sn->ctx;
+    IPG_HIP(hipSetDevice(ctx->device));
+    check_species_host(sn, sys, appear);
+    DevSystem s = upload_system(ctx, sys, "snap_api");
+    SnapCall c = snap_prepare(sn, s, mode, upload_appear(ctx, appear, s.n), true);
+    const int nc = sn->host.ncoeff, M = nc * sn->host.nspecies;
+    if (s.n == 0) {
+      if (W) std::fill(W, W + 6 * M, 0.0);
+      return;
+    }
+    double* dBo = scratch_t<double>(ctx, "snap_Bout", (size_t)nc * s.n);
+    double* Z = run_descriptors_fp64(sn, c, nullptr, dBo, nullptr, nullptr, dB || W);
+    if (B) d2h(ctx, B, dBo, sizeof(double) * nc * s.n);
+    if (dB || W) {
+      double* dbl = run_dblist(sn, c, Z);
+      if (dB) {
+        double* d = scratch_t<double>(ctx, "snap_dB", 3 * (size_t)M * s.n);
+        gather_db(sn, c, dbl, d);
+        d2h(ctx, dB, d, sizeof(double) * 3 * M * s.n);
+      }
+      if (W) {
+        double* Wa = scratch_t<double>(ctx, "snap_Wa", 6 * (size_t)M * s.n);
+        double* Wd = scratch_t<double>(ctx, "snap_W", 6 * (size_t)M);
+        launch(ctx, "snap_w_atom", [&] {
+          k_w_atom<<<grid_for(s.n * (int64_t)M, 256), 256, 0, ctx->stream>>>(sn->dev, c.R, s.pos, dbl, Wa);
+        });
+        reduce_rows(ctx, Wa, s.n, 6 * M, Wd, false);
+        d2h(ctx, W, Wd, sizeof(double) * 6 * M);
+      }
+    }
+    sync(ctx);
+  });
+}
+
+int ipg_snap_force_descriptors(ipg_snap* sn, const ipg_system* sys, int mode, const int32_t* appear, double* dB) {
+  if (!dB) return guard([] { fail(IPG_ERR_INVALID, "dB is NULL"); });
+  return ipg_snap_all_descriptors(sn, sys, mode, appear, nullptr, dB, nullptr);
+}
+
+int ipg_snap_energy_forces(ipg_snap* sn, const ipg_system* sys, int mode, const int32_t* appear, const double* beta,
+                           int64_t nbeta, double beta0, int path, double* energy, double* forces) {
+  return guard([&] {
+    IPG_REQUIRE(sn && sys, "NULL argument");
+    IPG_REQUIRE(std::isfinite(beta0), "beta0 must be finite");
+    ipg_ctx* ctx = sn->ctx;
+    IPG_HIP(hipSetDevice(ctx->device));
+    check_species_host(sn, sys, appear);
+    const double* db = expand_beta(sn, beta, nbeta);
+    DevSystem s = upload_system(ctx, sys, "snap_api");
+    SnapCall c = snap_prepare(sn, s, mode, upload_appear(ctx, appear, s.n), true);
+    double* out = scratch_t<double>(ctx, "snap_api_out", 1 + 3 * std::max<int64_t>(s.n, 1));
+    energy_forces_dev(sn, c, db, beta0, path, out, forces ? out + 1 : nullptr);
+    if (energy) d2h(ctx, energy, out, sizeof(double));
+    if (forces) d2h(ctx, forces, out + 1, sizeof(double) * 3 * s.n);
+    sync(ctx);
+  });
+}
+
+int ipg_snap_energy_forces_dev(ipg_snap* sn, const ipg_system* sys, int mode, const double* beta, double beta0,
+                               int path, double* energy, double* forces) {
+  return guard([&] {
+    IPG_REQUIRE(sn && sys && beta, "NULL argument");
+    ipg_ctx* ctx = sn->ctx;
+    IPG_HIP(hipSetDevice(ctx->device));
+    DevSystem s = make_dev_system(sys);
+    SnapCall c = snap_prepare(sn, s, mode, nullptr, true);
+    energy_forces_dev(sn, c, beta, beta0, path, energy, forces);
+  });
+}
+
+int ipg_snap_force_vjp(ipg_snap* sn, const ipg_system* sys, int mode, const int32_t* appear, const double* beta,
+                       int64_t nbeta, const double* gF, int precision, double* gbeta, double* gpos) {
+  return guard([&] {
+    IPG_REQUIRE(sn && sys && gF, "NULL argument");
+    IPG_REQUIRE(precision == IPG_FP64 || precision == IPG_FP32, "precision must be IPG_FP64 or IPG_FP32");
+    ipg_ctx* ctx = sn->ctx;
+    IPG_HIP(hipSetDevice(ctx->device));
+    check_species_host(sn, sys, appear);
+    for (int64_t i = 0; i < 3 * sys->natoms; i++) IPG_REQUIRE(std::isfinite(gF[i]), "gF must be finite");
+    const double* db = expand_beta(sn, beta, nbeta);
+    DevSystem s = upload_system(ctx, sys, "snap_api");
+    SnapCall c = snap_prepare(sn, s, mode, upload_appear(ctx, appear, s.n), true);
+    const SnapDev& P = sn->dev;
+    const int64_t n = s.n;
+    const int nc = P.ncoeff, M = nc * P.nspecies;
+    if (n == 0) {
+      if (gbeta) std::fill(gbeta, gbeta + nbeta, 0.0);
+      return;
+    }
+    double* tg = scratch_t<double>(ctx, "snap_vjp_tang", 3 * n);
+    h2d(ctx, tg, gF, sizeof(double) * 3 * n);
+    c.R.tang = tg;
+    double* dedr = scratch_t<double>(ctx, "snap_dedr", 3 * std::max<int64_t>(c.R.npairs, 1));
+    double* dedr_t = scratch_t<double>(ctx, "snap_dedr_t", 3 * std::max<int64_t>(c.R.npairs, 1));
+    double* tanB = scratch_t<double>(ctx, "snap_tanB", (size_t)M * n);
+    if (precision == IPG_FP64) {
+      using S = Dual<double>;
+      S* U = scratch_t<S>(ctx, "snap_Ud64", 2 * (size_t)P.idxu_max * n);
+      S* Y = scratch_t<S>(ctx, "snap_Yd64", 2 * (size_t)P.idxu_max * n);
+      double* yc = scratch_t<double>(ctx, "snap_yc64", (size_t)P.nspecies * P.nblk);
+      snap_launch_ycoef<double>(ctx, P, db, yc);
+      snap_launch_ui<S>(ctx, P, c.R, U);
+      snap_launch_zby<S>(ctx, P, c.R, U, nullptr, yc, nullptr, nullptr, Y, nullptr, tanB);
+      snap_launch_dedr<S>(ctx, P, c.R, Y, dedr, dedr_t);
+    } else {
+      using S = Dual<float>;
+      S* U = scratch_t<S>(ctx, "snap_Ud32", 2 * (size_t)P.idxu_max * n);
+      S* Y = scratch_t<S>(ctx, "snap_Yd32", 2 * (size_t)P.idxu_max * n);
+      float* yc = scratch_t<float>(ctx, "snap_yc32", (size_t)P.nspecies * P.nblk);
+      snap_launch_ycoef<float>(ctx, P, db, yc);
+      snap_launch_ui<S>(ctx, P, c.R, U);
+      snap_launch_zby<S>(ctx, P, c.R, U, nullptr, yc, nullptr, nullptr, Y, nullptr, tanB);
+      snap_launch_dedr<S>(ctx, P, c.R, Y, dedr, dedr_t);
+    }
+    double* gp = scratch_t<double>(ctx, "snap_vjp_gpos", 3 * n);
+    launch(ctx, "snap_force_gather", [&] {
+      k_force_gather<<<grid_for(n, 256), 256, 0, ctx->stream>>>(n, c.R.off, c.R.roff, c.R.rpair, dedr_t, gp);
+    });
+    double* gb = scratch_t<double>(ctx, "snap_vjp_gbeta", M);
+    reduce_rows(ctx, tanB, n, M, gb, false);
+    std::vector<double> hgb(M);
+    d2h(ctx, hgb.data(), gb, sizeof(double) * M);
+    if (gpos) d2h(ctx, gpos, gp, sizeof(double) * 3 * n);
+    sync(ctx);
+    if (gbeta) {
+      // gbeta_c = sum_k gF_k . dB[k][c] = -d/de sum_(i in block) B_i[c](x + e gF)
+      if (nbeta == nc) {
+        for (int cc = 0; cc < nc; cc++) {
+          double v = 0.0;
+          for (int sp = 0; sp < P.nspecies; sp++) v += hgb[sp * nc + cc];
+          gbeta[cc] = -v;
+        }
+      } else {
+        for (int m = 0; m < M; m++) gbeta[m] = -hgb[m];
+      }
+    }
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,73 @@
+"""The C-ABI library loads and exports every entry point include/ipgpu.h
+declares (no GPU calls), and the host-side logic of the Python mirror."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import REPO
+
+
+def header_symbols():
+    txt = open(os.path.join(REPO, "include", "ipgpu.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(ipg_\w+)\s*\(", txt, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    import ipgpu
+
+    L = ipgpu.load()
+    syms = header_symbols()
+    assert len(syms) >= 25
+    for s in syms:
+        assert hasattr(L, s), s
+    assert sorted(ipgpu._lib.EXPORTED) == syms
+    assert L.ipg_get_version() == 100
+
+
+def test_status_codes_without_gpu_work():
+    import ipgpu
+
+    L = ipgpu.load()
+    # NULL arguments are rejected before any device call
+    assert L.ipg_ctx_create(0, None) == ipgpu._lib.IPG_ERR_INVALID
+    assert b"NULL" in L.ipg_last_error()
+    assert L.ipg_nlist_info(None, None, None, None, None) == ipgpu._lib.IPG_ERR_INVALID
+
+
+def test_units_and_system():
+    import ipgpu
+
+    assert ipgpu.angstrom == pytest.approx(1.8897261246257702)
+    assert ipgpu.eV == pytest.approx(1 / 27.211386245988)
+    s = ipgpu.AtomsSystem([[1, 2, 3]], np.diag([4.0, 5, 6]), (1, 0, 1), ["X"], unit="angstrom")
+    np.testing.assert_allclose(s.austrip_positions(), np.array([[1, 2, 3]]) * ipgpu.angstrom)
+    np.testing.assert_allclose(s.box, [4, 5, 6])
+    with pytest.raises(ValueError):
+        ipgpu.AtomsSystem([[0, 0, 0]], [[1, 1, 0], [0, 1, 0], [0, 0, 1]], (1, 1, 1), ["X"])
+    s2 = ipgpu.AtomsSystem(np.zeros((4, 3)), [1, 1, 1], (1, 1, 1), ["B", "A", "B", "C"])
+    assert s2.unique_symbols() == ["B", "A", "C"]
+
+
+def test_linear_combination_algebra_and_terms():
+    import ipgpu
+
+    lj = ipgpu.LennardJones(1.0, 2.0, 5.0, ["A"])
+    mo = ipgpu.Morse(0.1, 0.5, 1.0, 4.0, ["A", "B"])
+    p = 2.0 * lj - mo / 4.0 + lj
+    terms = p.flat_terms()
+    assert [c for _, c in terms] == [2.0, -0.25, 1.0]
+    assert p.get_rcutoff() == 5.0 and p.get_species() is None
+    t = lj._term(3.0, ["B", "A"])
+    assert t.species_mask == 0b10 and t.coeff == 3.0 and t.kind == ipgpu._lib.IPG_PAIR_LJ
+    assert ipgpu.ZBL(1, 2, 1.0, 3.0, None)._term(1.0, ["A"]).species_mask == (1 << 64) - 1
+
+
+def test_snap_num_coeffs():
+    from ipgpu.snap import get_num_snap_coeffs
+
+    assert [get_num_snap_coeffs(J, 1, False) for J in (4, 6, 8)] == [14, 30, 55]
+    assert get_num_snap_coeffs(3, 1, False) == 8
+    assert get_num_snap_coeffs(4, 2, True) == 14 * 8

@@ -1,0 +1,150 @@
+"""GPU neighbor lists vs the oracle: bit-exact neighbor sets and displacements
+(src/nnlist.jl:23-46, utl.jl:38-101)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+INF = np.inf
+
+
+def vectors():
+    with open(os.path.join(GOLDEN, "reference_vectors.json")) as f:
+        return json.load(f)
+
+
+def gpu_list(ipg, pos, box, per, rcut, full=False):
+    from ipgpu.nnlist import neighborlist_arrays
+    from ipgpu._lib import IPG_LIST_FULL, IPG_LIST_HALF
+
+    return neighborlist_arrays(pos, box, per, rcut, IPG_LIST_FULL if full else IPG_LIST_HALF)
+
+
+def oracle_list(orc, pos, box, per, rcut, full=False):
+    L = [box[d] if per[d] else INF for d in range(3)]
+    return orc.neighborlist(pos, L, rcut, kind=orc.LIST_FULL if full else orc.LIST_HALF)
+
+
+def assert_same(a, b):
+    off_a, j_a, d_a = a
+    off_b, j_b, d_b = b
+    np.testing.assert_array_equal(off_a, off_b)
+    np.testing.assert_array_equal(j_a, j_b)
+    if d_a is not None and d_b is not None:
+        np.testing.assert_array_equal(d_a, d_b)  # bit-exact displacements
+
+
+def test_reference_unit_vector(ipg):
+    v = vectors()["nnlist_unit"]
+    s = ipg.AtomsSystem(v["positions_bohr"], v["box_bohr"], v["periodic"], ["Ar"] * 3, unit="bohr")
+    nl = ipg.neighborlist(s, v["rcutoff"])
+    assert len(nl) == 3
+    assert [list(j + 1) for j in nl.j] == v["j_1based"]
+    for ri, tr in zip(nl.r, v["r"]):
+        if tr:
+            np.testing.assert_allclose(ri, np.array(tr), rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("seed,n,per,rcut", [
+    (0, 300, (True, True, True), 2.5),
+    (1, 500, (True, False, True), 3.1),
+    (2, 400, (False, False, False), 2.0),
+    (3, 64, (True, True, True), 7.9),      # rcut > L/2: minimum image only
+    (4, 200, (True, True, False), 1.0),
+])
+def test_random_half_and_full(ipg, orc, seed, n, per, rcut):
+    rng = np.random.default_rng(seed)
+    box = np.array([9.0, 11.0, 10.0]) if n > 64 else np.array([5.0, 6.0, 7.0])
+    pos = rng.random((n, 3)) * box
+    pos[::7] += rng.integers(-3, 4, size=pos[::7].shape) * box  # unwrapped images
+    for full in (False, True):
+        a = gpu_list(ipg, pos, box, per, rcut, full)
+        b = oracle_list(orc, pos, box, per, rcut, full)
+        assert_same(a, b)
+
+
+def test_edge_cases(ipg, orc):
+    box = np.array([4.0, 4.0, 4.0])
+    # empty and single-atom systems
+    off, j, d = gpu_list(ipg, np.zeros((0, 3)), box, (True,) * 3, 2.0)
+    assert len(off) == 1 and off[0] == 0 and len(j) == 0
+    off, j, d = gpu_list(ipg, np.array([[1.0, 2.0, 3.0]]), box, (True,) * 3, 2.0)
+    assert list(off) == [0, 0]
+    # all atoms on top of each other
+    pos = np.tile([[1.0, 1.0, 1.0]], (20, 1))
+    assert_same(gpu_list(ipg, pos, box, (True,) * 3, 0.5), oracle_list(orc, pos, box, (True,) * 3, 0.5))
+    # pairs exactly at the half box and exactly at rcut
+    pos = np.array([[0.0, 0, 0], [2.0, 0, 0], [0, 1.5, 0], [0, 0, 3.0]])
+    for rc in (1.5, 2.0, 1.0):
+        for full in (False, True):
+            assert_same(gpu_list(ipg, pos, box, (True,) * 3, rc, full), oracle_list(orc, pos, box, (True,) * 3, rc, full))
+
+
+def test_invalid_inputs(ipg):
+    with pytest.raises(ipg.IPGError):
+        gpu_list(ipg, np.zeros((3, 3)), np.array([1.0, 1.0, 1.0]), (True,) * 3, -1.0)
+    with pytest.raises(ipg.IPGError):
+        gpu_list(ipg, np.array([[np.nan, 0, 0]]), np.array([1.0, 1.0, 1.0]), (True,) * 3, 1.0)
+    with pytest.raises(ipg.IPGError):
+        gpu_list(ipg, np.zeros((2, 3)), np.array([0.0, 1.0, 1.0]), (True,) * 3, 1.0)
+
+
+def test_large_periodic_lattice(ipg, orc):
+    from ipgpu import workloads
+
+    s, p = workloads.lj_argon_fcc()
+    pos, box = s.austrip_positions(), s.austrip_box()
+    a = gpu_list(ipg, pos, box, (True,) * 3, p.rcutoff)
+    b = orc.neighborlist(pos, box, p.rcutoff, kind=orc.LIST_HALF, cells=True)
+    assert_same(a, b)
+    assert abs(len(a[1]) / 1000 - 35.4) < 3
+
+
+def _snap_case(ipg, orc, mode, radii, nspecies, seed, n=120, a=3.16, rcutfac=2.0, dense=False):
+    rng = np.random.default_rng(seed)
+    from ipgpu import workloads
+
+    if dense:
+        L = np.array([6.0, 6.0, 6.0])
+        X = rng.random((n, 3)) * L
+    else:
+        X, L = workloads.bcc_positions(4, 4, 4, a)
+        X = X[:n] + rng.normal(scale=0.05, size=(n, 3))
+    names = ["W", "Mo", "Ta"][:nspecies]
+    syms = [names[k] for k in rng.integers(0, nspecies, n)]
+    s = ipg.AtomsSystem(X, L, (True,) * 3, syms, unit="angstrom")
+    sn = ipg.snap.SNAP(n, 4, names, rcutfac, 0.0, 0.99363, radii, [1.0] * nspecies, list_mode=mode)
+    g = ipg.snap.snap_neighborlist(s, sn)
+    o = orc.Snap(4, nspecies, rcutfac, 0.0, 0.99363, radii, [1.0] * nspecies)
+    sp = np.array([names.index(x) for x in syms], dtype=np.int32)
+    uniq = s.unique_symbols()
+    ap = np.array([uniq.index(x) for x in syms], dtype=np.int32)
+    off, j, R, r = o.neighborlist(X, L, sp, appear=ap,
+                                  mode=orc.SNAP_LIST_REFERENCE if mode == "reference" else orc.SNAP_LIST_CUTOFF)
+    np.testing.assert_array_equal(g.offsets, off)
+    np.testing.assert_array_equal(g.j, j)
+    np.testing.assert_array_equal(g.r, r)
+    np.testing.assert_allclose(g.R, R, rtol=2e-16, atol=0)
+    return len(j)
+
+
+@pytest.mark.parametrize("mode", ["reference", "cutoff"])
+def test_snap_list_equal_radii(ipg, orc, mode):
+    npairs = _snap_case(ipg, orc, mode, [1.0, 1.0], 2, 7)
+    if mode == "reference":
+        assert npairs == 0  # inrange radius sqrt(4.0) = 2.0 < bcc nearest distance 2.74
+    else:
+        assert npairs > 0
+
+
+@pytest.mark.parametrize("mode", ["reference", "cutoff"])
+def test_snap_list_unequal_radii(ipg, orc, mode):
+    # dense random atoms so that the per-species cutoffs split the inrange
+    # result: exercises the as-is element lookups of utl.jl:70,86
+    npairs = _snap_case(ipg, orc, mode, [0.3, 1.5, 0.9], 3, 11, rcutfac=1.0, dense=True)
+    assert npairs > 0
