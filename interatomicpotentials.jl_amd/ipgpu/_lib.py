@@ -80,7 +80,7 @@ class SnapParams(C.Structure):
 
 
 _lib = None
-_lock = threading.Lock()
+_lock = threading.RLock()  # default_context() calls load() while holding it
 
 
 def load():

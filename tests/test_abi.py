@@ -71,3 +71,18 @@ def test_snap_num_coeffs():
     assert [get_num_snap_coeffs(J, 1, False) for J in (4, 6, 8)] == [14, 30, 55]
     assert get_num_snap_coeffs(3, 1, False) == 8
     assert get_num_snap_coeffs(4, 2, True) == 14 * 8
+
+
+def test_default_context_returns_promptly():
+    """Regression: creating the default context must not deadlock (it either
+    returns a context or raises IPGError when no GPU is present)."""
+    import subprocess
+    import sys
+
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "import ipgpu\n"
+            "try:\n    ipgpu._lib.default_context(); print('ctx')\n"
+            "except ipgpu.IPGError as e:\n    print('err', e.code)\n") % os.path.join(REPO, "interatomicpotentials.jl_amd")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.startswith("ctx") or r.stdout.startswith("err -3")
