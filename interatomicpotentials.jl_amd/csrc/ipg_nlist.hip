@@ -11,6 +11,8 @@
 // correctly rounded sqrt is <= rcut; this equals sqrt(s) <= rcut exactly.
 #include "ipg_nlist.h"
 
+#include <algorithm>
+
 namespace ipg {
 
 struct Grid {
@@ -43,38 +45,40 @@ __global__ void k_cell_assign(const double* __restrict__ pos, int64_t n, Grid g,
 }
 
 __global__ void k_cell_place(int64_t n, const int32_t* __restrict__ cellof, const int64_t* __restrict__ cstart,
-                             int32_t* __restrict__ cfill, int32_t* __restrict__ catom) {
+                             int32_t* __restrict__ cfill, int32_t* __restrict__ ctmp) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int32_t c = cellof[i];
   int32_t slot = atomicAdd(&cfill[c], 1);
-  catom[cstart[c] + slot] = (int32_t)i;
+  ctmp[cstart[c] + slot] = (int32_t)i;
 }
 
-// Put each cell's atoms in ascending index order and gather positions in
-// cell order (double4, w unused) for locality of the candidate scans.
-__global__ void k_cell_sort(int64_t ncell, const int64_t* __restrict__ cstart, int32_t* __restrict__ catom,
-                            const double* __restrict__ pos, double4* __restrict__ spos) {
-  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= ncell) return;
-  int64_t a = cstart[c], b = cstart[c + 1];
-  for (int64_t k = a + 1; k < b; k++) {
-    int32_t v = catom[k];
-    int64_t m = k - 1;
-    while (m >= a && catom[m] > v) {
-      catom[m + 1] = catom[m];
-      m--;
-    }
-    catom[m + 1] = v;
-  }
-  for (int64_t k = a; k < b; k++) {
-    int32_t i = catom[k];
-    spos[k] = make_double4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], 0.0);
-  }
+// Order each cell's atoms by index (rank of every entry within its cell, so
+// the layout does not depend on the atomic slot order) and gather positions
+// in cell order: exact doubles for the membership test and FP32 wrapped
+// coordinates for the prefilter.
+__global__ void k_cell_rank(int64_t n, const int32_t* __restrict__ ctmp, const int32_t* __restrict__ cellof,
+                            const int64_t* __restrict__ cstart, const double* __restrict__ pos, Grid g,
+                            int32_t* __restrict__ catom, double4* __restrict__ spos, float4* __restrict__ sposf) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int32_t i = ctmp[k];
+  const int32_t c = cellof[i];
+  const int64_t a = cstart[c], b = cstart[c + 1];
+  int64_t r = 0;
+  for (int64_t m = a; m < b; m++) r += ctmp[m] < i;
+  const int64_t dst = a + r;
+  catom[dst] = i;
+  const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+  spos[dst] = make_double4(x, y, z, 0.0);
+  double w[3] = {x, y, z};
+  for (int d = 0; d < 3; d++) w[d] = g.box.per[d] ? w[d] - floor(w[d] / g.box.L[d]) * g.box.L[d] : w[d] - g.lo[d];
+  sposf[dst] = make_float4((float)w[0], (float)w[1], (float)w[2], 0.f);
 }
 
 template <typename F>
-__device__ __forceinline__ void scan_cells(const Grid& g, int32_t cell, const int64_t* __restrict__ cstart, F&& f) {
+__device__ __forceinline__ void for_each_cell(const Grid& g, int32_t cell, const int64_t* __restrict__ cstart,
+                                              F&& f) {
   int cc[3];
   cc[0] = cell % g.nc[0];
   cc[1] = (cell / g.nc[0]) % g.nc[1];
@@ -108,9 +112,8 @@ __device__ __forceinline__ void scan_cells(const Grid& g, int32_t cell, const in
           if (g.box.per[0]) xx = (x + g.nc[0]) % g.nc[0];
           else if (x < 0 || x >= g.nc[0]) continue;
         }
-        int64_t cid = ((int64_t)zz * g.nc[1] + yy) * g.nc[0] + xx;
-        int64_t s1 = cstart[cid + 1];
-        for (int64_t s = cstart[cid]; s < s1; s++) f(s);
+        const int64_t cid = ((int64_t)zz * g.nc[1] + yy) * g.nc[0] + xx;
+        f(cstart[cid], cstart[cid + 1]);
       }
     }
   }
@@ -122,65 +125,93 @@ __device__ __forceinline__ bool keep_pair(int kind, int32_t i, int32_t j) {
   return true;  // KIND_FULL_SELF
 }
 
-__global__ void k_list_count(int64_t n, Grid g, const int32_t* __restrict__ catom, const double4* __restrict__ spos,
-                             const int32_t* __restrict__ cellof, const int64_t* __restrict__ cstart, double smax,
-                             int kind, int32_t* __restrict__ cnt) {
-  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  int32_t i = catom[k];
-  double4 pi = spos[k];
-  int32_t c = cellof[i];
+struct Prefilter {
+  float L[3], invL[3];
+  float r2;  // FP32 squared-distance bound that no member can exceed
+};
+
+// LP lanes per atom scan the surrounding cells; each cell's candidates are
+// taken LP at a time.  A cheap FP32 minimum-image distance rejects far
+// candidates; the rest get the exact FP64 PeriodicEuclidean test.  Hits are
+// placed with a ballot, so the output order is fixed (cell order, then slot).
+template <int LP, bool FILL>
+__global__ __launch_bounds__(256) void k_list_scan(int64_t n, Grid g, Prefilter pf, const int32_t* __restrict__ catom,
+                                                   const double4* __restrict__ spos, const float4* __restrict__ sposf,
+                                                   const int32_t* __restrict__ cellof,
+                                                   const int64_t* __restrict__ cstart, double smax, int kind,
+                                                   int32_t* __restrict__ cnt, const int64_t* __restrict__ off,
+                                                   int32_t* __restrict__ nbr) {
+  const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t k = gt / LP;
+  const int l = (int)(gt % LP);
+  const int lane = threadIdx.x & 63;
+  if (k >= n) return;  // whole groups leave together
+  const uint64_t gmask = (LP == 64 ? ~0ull : ((1ull << LP) - 1ull)) << (lane - l);
+  const uint64_t below = (1ull << lane) - 1ull;
+  const int32_t i = catom[k];
+  const double4 pi = spos[k];
+  const float4 qi = sposf[k];
+  int64_t w = FILL ? off[i] : 0;
   int count = 0;
-  scan_cells(g, c, cstart, [&](int64_t s) {
-    int32_t j = catom[s];
-    if (!keep_pair(kind, i, j)) return;
-    double4 pj = spos[s];
-    if (ipg_pe_sumsq(g.box, pi.x, pi.y, pi.z, pj.x, pj.y, pj.z) <= smax) count++;
+  for_each_cell(g, cellof[i], cstart, [&](int64_t s0, int64_t s1) {
+    for (int64_t s = s0; s < s1; s += LP) {
+      const int64_t sl = s + l;
+      bool hit = false;
+      int32_t j = -1;
+      if (sl < s1) {
+        j = catom[sl];
+        if (keep_pair(kind, i, j)) {
+          const float4 qj = sposf[sl];
+          float dx = qj.x - qi.x, dy = qj.y - qi.y, dz = qj.z - qi.z;
+          if (g.box.per[0]) dx -= pf.L[0] * rintf(dx * pf.invL[0]);
+          if (g.box.per[1]) dy -= pf.L[1] * rintf(dy * pf.invL[1]);
+          if (g.box.per[2]) dz -= pf.L[2] * rintf(dz * pf.invL[2]);
+          if (dx * dx + dy * dy + dz * dz <= pf.r2) {
+            const double4 pj = spos[sl];
+            hit = ipg_pe_sumsq(g.box, pi.x, pi.y, pi.z, pj.x, pj.y, pj.z) <= smax;
+          }
+        }
+      }
+      const uint64_t m = __ballot(hit) & gmask;
+      if (FILL) {
+        if (hit) nbr[w + __popcll(m & below)] = j;
+        w += __popcll(m);
+      } else {
+        count += __popcll(m);
+      }
+    }
   });
-  cnt[i] = count;
+  if (!FILL && l == 0) cnt[i] = count;
 }
 
-__global__ void k_list_fill(int64_t n, Grid g, const int32_t* __restrict__ catom, const double4* __restrict__ spos,
-                            const int32_t* __restrict__ cellof, const int64_t* __restrict__ cstart, double smax,
-                            int kind, const int64_t* __restrict__ off, int32_t* __restrict__ nbr) {
-  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  int32_t i = catom[k];
-  double4 pi = spos[k];
-  int32_t c = cellof[i];
-  int64_t w = off[i];
-  scan_cells(g, c, cstart, [&](int64_t s) {
-    int32_t j = catom[s];
-    if (!keep_pair(kind, i, j)) return;
-    double4 pj = spos[s];
-    if (ipg_pe_sumsq(g.box, pi.x, pi.y, pi.z, pj.x, pj.y, pj.z) <= smax) nbr[w++] = j;
-  });
+// In-segment rank sort: out[off + rank] = in[q], rank = number of smaller
+// entries of the same segment (values within a segment are distinct).
+__global__ void k_seg_rank_sort(int64_t total, int64_t nseg, const int64_t* __restrict__ off,
+                                const int32_t* __restrict__ in, int32_t* __restrict__ out) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= total) return;
+  int64_t lo = 0, hi = nseg;  // find the segment with off[seg] <= q < off[seg+1]
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (off[mid] <= q) lo = mid;
+    else hi = mid;
+  }
+  const int32_t v = in[q];
+  int64_t r = 0;
+  for (int64_t m = off[lo]; m < off[lo + 1]; m++) r += in[m] < v;
+  out[off[lo] + r] = v;
 }
 
-__global__ void k_list_finish(int64_t n, const int64_t* __restrict__ off, int32_t* __restrict__ nbr, int sorted,
-                              const double* __restrict__ pos, IpgBox box, double* __restrict__ disp) {
+__global__ void k_list_disp(int64_t n, const int64_t* __restrict__ off, const int32_t* __restrict__ nbr,
+                            const double* __restrict__ pos, IpgBox box, double* __restrict__ disp) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  int64_t a = off[i], b = off[i + 1];
-  if (sorted) {
-    for (int64_t k = a + 1; k < b; k++) {
-      int32_t v = nbr[k];
-      int64_t m = k - 1;
-      while (m >= a && nbr[m] > v) {
-        nbr[m + 1] = nbr[m];
-        m--;
-      }
-      nbr[m + 1] = v;
-    }
-  }
-  if (disp) {
-    double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
-    for (int64_t k = a; k < b; k++) {
-      int32_t j = nbr[k];
-      disp[3 * k] = ipg_disp1(xi, pos[3 * j], box.L[0], box.per[0]);
-      disp[3 * k + 1] = ipg_disp1(yi, pos[3 * j + 1], box.L[1], box.per[1]);
-      disp[3 * k + 2] = ipg_disp1(zi, pos[3 * j + 2], box.L[2], box.per[2]);
-    }
+  const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+  for (int64_t k = off[i]; k < off[i + 1]; k++) {
+    const int32_t j = nbr[k];
+    disp[3 * k] = ipg_disp1(xi, pos[3 * j], box.L[0], box.per[0]);
+    disp[3 * k + 1] = ipg_disp1(yi, pos[3 * j + 1], box.L[1], box.per[1]);
+    disp[3 * k + 2] = ipg_disp1(zi, pos[3 * j + 2], box.L[2], box.per[2]);
   }
 }
 
@@ -315,36 +346,67 @@ static DevList build_list_impl(ipg_ctx* ctx, const DevSystem& s, double rcut, in
     return L;
   }
   Grid g = make_grid(ctx, s, rcut);
-  double smax = sqrt_le_threshold(rcut);
+  const double smax = sqrt_le_threshold(rcut);
+  Prefilter pf;
+  double ext = 0.0;
+  for (int d = 0; d < 3; d++) {
+    pf.L[d] = g.box.per[d] ? (float)g.box.L[d] : 0.f;
+    pf.invL[d] = g.box.per[d] ? (float)(1.0 / g.box.L[d]) : 0.f;
+    ext = std::max(ext, g.w[d] * g.nc[d]);
+  }
+  // FP32 coordinate and arithmetic error bound per component, then a
+  // squared-distance bound no true member can exceed
+  const double err = 8.0 * ext * std::ldexp(1.0, -23) + 1e-6 * rcut;
+  const double rr = rcut + 2.0 * std::sqrt(3.0) * err;
+  pf.r2 = (float)(rr * rr * (1.0 + 1e-5));
   int32_t* cellof = scratch_t<int32_t>(ctx, "nl_cellof", s.n);
   int32_t* ccount = scratch_t<int32_t>(ctx, "nl_ccount", 2 * g.ncell + 2);
   int32_t* cfill = ccount + g.ncell + 1;
   int64_t* cstart = scratch_t<int64_t>(ctx, "nl_cstart", g.ncell + 1);
+  int32_t* ctmp = scratch_t<int32_t>(ctx, "nl_ctmp", s.n);
   int32_t* catom = scratch_t<int32_t>(ctx, "nl_catom", s.n);
   double4* spos = scratch_t<double4>(ctx, "nl_spos", s.n);
+  float4* sposf = scratch_t<float4>(ctx, "nl_sposf", s.n);
   int32_t* cnt = scratch_t<int32_t>(ctx, "nl_cnt", s.n);
   zero(ctx, ccount, sizeof(int32_t) * (2 * g.ncell + 2));
-  unsigned gb = grid_for(s.n, 256);
+  const unsigned gb = grid_for(s.n, 256);
   launch(ctx, "cell_assign", [&] { k_cell_assign<<<gb, 256, 0, ctx->stream>>>(s.pos, s.n, g, cellof, ccount); });
   exclusive_scan(ctx, ccount, g.ncell, cstart, false);
-  launch(ctx, "cell_place", [&] { k_cell_place<<<gb, 256, 0, ctx->stream>>>(s.n, cellof, cstart, cfill, catom); });
-  launch(ctx, "cell_sort", [&] {
-    k_cell_sort<<<grid_for(g.ncell, 256), 256, 0, ctx->stream>>>(g.ncell, cstart, catom, s.pos, spos);
+  launch(ctx, "cell_place", [&] { k_cell_place<<<gb, 256, 0, ctx->stream>>>(s.n, cellof, cstart, cfill, ctmp); });
+  launch(ctx, "cell_rank", [&] {
+    k_cell_rank<<<gb, 256, 0, ctx->stream>>>(s.n, ctmp, cellof, cstart, s.pos, g, catom, spos, sposf);
   });
-  launch(ctx, "nlist_count", [&] {
-    k_list_count<<<gb, 256, 0, ctx->stream>>>(s.n, g, catom, spos, cellof, cstart, smax, kind, cnt);
-  });
+  // lanes per atom from the expected candidate count
+  int ncellv = 1;
+  for (int d = 0; d < 3; d++) ncellv *= g.full[d] ? g.nc[d] : std::min(3, g.nc[d]);
+  const double cand = (double)s.n / (double)g.ncell * ncellv;
+  const int LP = cand >= 192 ? 16 : (cand >= 48 ? 8 : 4);
+  const unsigned gl = grid_for(s.n * LP, 256);
+  auto scan = [&](auto lp, auto fill, int64_t* offp, int32_t* nbrp) {
+    constexpr int L = decltype(lp)::value;
+    constexpr bool F = decltype(fill)::value;
+    k_list_scan<L, F><<<gl, 256, 0, ctx->stream>>>(s.n, g, pf, catom, spos, sposf, cellof, cstart, smax, kind, cnt,
+                                                   offp, nbrp);
+  };
+  auto dispatch = [&](auto fill, int64_t* offp, int32_t* nbrp) {
+    if (LP == 16) scan(std::integral_constant<int, 16>(), fill, offp, nbrp);
+    else if (LP == 8) scan(std::integral_constant<int, 8>(), fill, offp, nbrp);
+    else scan(std::integral_constant<int, 4>(), fill, offp, nbrp);
+  };
+  launch(ctx, "nlist_count", [&] { dispatch(std::false_type(), nullptr, nullptr); });
   int64_t np = exclusive_scan(ctx, cnt, s.n, L.off, true);
   L.npairs = np;
   L.nbr = (int32_t*)alloc("nbr", sizeof(int32_t) * (np > 0 ? np : 1));
   L.disp = want_disp ? (double*)alloc("disp", sizeof(double) * 3 * (np > 0 ? np : 1)) : nullptr;
-  launch(ctx, "nlist_fill", [&] {
-    k_list_fill<<<gb, 256, 0, ctx->stream>>>(s.n, g, catom, spos, cellof, cstart, smax, kind, L.off, L.nbr);
-  });
-  if (sorted || want_disp)
-    launch(ctx, "nlist_finish", [&] {
-      k_list_finish<<<gb, 256, 0, ctx->stream>>>(s.n, L.off, L.nbr, sorted ? 1 : 0, s.pos, s.box, L.disp);
+  int32_t* fill_to = L.nbr;
+  if (sorted && np > 0) fill_to = scratch_t<int32_t>(ctx, "nl_unsorted", np);
+  launch(ctx, "nlist_fill", [&] { dispatch(std::true_type(), L.off, fill_to); });
+  if (sorted && np > 0)
+    launch(ctx, "nlist_sort", [&] {
+      k_seg_rank_sort<<<grid_for(np, 256), 256, 0, ctx->stream>>>(np, s.n, L.off, fill_to, L.nbr);
     });
+  if (want_disp && np > 0)
+    launch(ctx, "nlist_disp", [&] { k_list_disp<<<gb, 256, 0, ctx->stream>>>(s.n, L.off, L.nbr, s.pos, s.box, L.disp); });
   return L;
 }
 
@@ -435,27 +497,12 @@ __global__ void k_rev_count(int64_t np, const int32_t* __restrict__ nbr, int32_t
 }
 
 __global__ void k_rev_place(int64_t np, const int32_t* __restrict__ nbr, const int64_t* __restrict__ roff,
-                            int32_t* __restrict__ rfill, int32_t* __restrict__ rpair) {
+                            int32_t* __restrict__ rfill, int32_t* __restrict__ rtmp) {
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= np) return;
   int32_t j = nbr[q];
   int32_t slot = atomicAdd(&rfill[j], 1);
-  rpair[roff[j] + slot] = (int32_t)q;
-}
-
-__global__ void k_rev_sort(int64_t n, const int64_t* __restrict__ roff, int32_t* __restrict__ rpair) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int64_t a = roff[i], b = roff[i + 1];
-  for (int64_t k = a + 1; k < b; k++) {
-    int32_t v = rpair[k];
-    int64_t m = k - 1;
-    while (m >= a && rpair[m] > v) {
-      rpair[m + 1] = rpair[m];
-      m--;
-    }
-    rpair[m + 1] = v;
-  }
+  rtmp[roff[j] + slot] = (int32_t)q;
 }
 
 void build_center_and_reverse(ipg_ctx* ctx, int64_t n, int64_t npairs, const int64_t* off, const int32_t* nbr,
@@ -470,8 +517,9 @@ void build_center_and_reverse(ipg_ctx* ctx, int64_t n, int64_t npairs, const int
   if (npairs > 0) launch(ctx, "rev_count", [&] { k_rev_count<<<gp, 256, 0, ctx->stream>>>(npairs, nbr, rc); });
   exclusive_scan(ctx, rc, n, roff, false);
   if (npairs > 0) {
-    launch(ctx, "rev_place", [&] { k_rev_place<<<gp, 256, 0, ctx->stream>>>(npairs, nbr, roff, rf, rpair); });
-    launch(ctx, "rev_sort", [&] { k_rev_sort<<<gn, 256, 0, ctx->stream>>>(n, roff, rpair); });
+    int32_t* rtmp = scratch_t<int32_t>(ctx, (t + "_rtmp").c_str(), npairs);
+    launch(ctx, "rev_place", [&] { k_rev_place<<<gp, 256, 0, ctx->stream>>>(npairs, nbr, roff, rf, rtmp); });
+    launch(ctx, "rev_sort", [&] { k_seg_rank_sort<<<gp, 256, 0, ctx->stream>>>(npairs, n, roff, rtmp, rpair); });
   }
 }
 

@@ -19,13 +19,66 @@ namespace ipg {
 
 __host__ __device__ constexpr int ub_of(int j) { return j * (j + 1) * (2 * j + 1) / 6; }  // idxu_block[j]
 
-template <typename T>
-__device__ __forceinline__ T shfl_xor_s(T v, int o) {
-  return __shfl_xor(v, o, 64);
+// ---------------------------------------------------------------------------
+// Deterministic sums over groups of G = 16, 32 or 64 lanes: DPP within each
+// 16-lane row (xor 1, xor 2, half-row mirror, row mirror: every lane ends with
+// its row total), then readlane of the row totals.  All lanes of a group
+// receive bit-identical totals.
+// ---------------------------------------------------------------------------
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v) {
+  return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, true);
 }
-template <typename T>
-__device__ __forceinline__ Dual<T> shfl_xor_s(Dual<T> v, int o) {
-  return Dual<T>(__shfl_xor(v.v, o, 64), __shfl_xor(v.t, o, 64));
+template <int CTRL>
+__device__ __forceinline__ float dpp_v(float v) {
+  return __int_as_float(dpp_i<CTRL>(__float_as_int(v)));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_v(double v) {
+  const int lo = dpp_i<CTRL>(__double2loint(v)), hi = dpp_i<CTRL>(__double2hiint(v));
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float readlane_v(float v, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double readlane_v(double v, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                          __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+template <int G, typename T>
+__device__ __forceinline__ T group_sum_t(T v) {
+  v = v + dpp_v<0xB1>(v);   // quad_perm [1,0,3,2]
+  v = v + dpp_v<0x4E>(v);   // quad_perm [2,3,0,1]
+  v = v + dpp_v<0x141>(v);  // row_half_mirror
+  v = v + dpp_v<0x140>(v);  // row_mirror
+  if constexpr (G == 16) {
+    return v;
+  } else {
+    const T r0 = readlane_v(v, 0), r1 = readlane_v(v, 16), r2 = readlane_v(v, 32), r3 = readlane_v(v, 48);
+    if constexpr (G == 32) return (threadIdx.x & 32) ? r2 + r3 : r0 + r1;
+    else return (r0 + r1) + (r2 + r3);
+  }
+}
+template <int G, typename T>
+__device__ __forceinline__ T group_sum(T v) {
+  return group_sum_t<G, T>(v);
+}
+template <int G, typename T>
+__device__ __forceinline__ Dual<T> group_sum(Dual<T> v) {
+  return Dual<T>(group_sum_t<G, T>(v.v), group_sum_t<G, T>(v.t));
+}
+
+// Wider accumulator type: float -> double (so FP32 kernels keep FP64 sums).
+template <typename S> struct AccOf { using type = double; };
+template <typename T> struct AccOf<Dual<T>> { using type = Dual<double>; };
+template <typename S> __device__ __forceinline__ typename AccOf<S>::type to_acc(S x) { return (double)x; }
+template <typename T> __device__ __forceinline__ Dual<double> to_acc(Dual<T> x) {
+  return Dual<double>((double)x.v, (double)x.t);
+}
+template <typename S> __device__ __forceinline__ S from_acc(double x) { return (S)x; }
+template <typename S> __device__ __forceinline__ S from_acc(Dual<double> x) {
+  using T = typename BaseOf<S>::type;
+  return S((T)x.v, (T)x.t);
 }
 
 template <typename S>
@@ -86,24 +139,35 @@ __device__ __forceinline__ void cayley_klein(S x, S y, S z, double rcut, double 
   }
 }
 
-// sqrt(p/q) (prebuilt_arrays.jl:314-321); the arguments are compile-time
-// constants after unrolling, so this folds to an immediate.
-template <typename T>
-__device__ __forceinline__ T rootpq(int p, int q) {
-  return (T)__builtin_sqrt((double)p / (double)q);
+template <int V>
+using IC = std::integral_constant<int, V>;
+
+// Compile-time loop: f(IC<B>), f(IC<B+1>), ..., f(IC<E-1>).  Every index the
+// Wigner recursion touches is a template constant, so the register rows are
+// indexed statically and sqrt(p/q) folds to an immediate.
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(IC<B>{});
+    static_for<B + 1, E>(static_cast<F&&>(f));
+  }
 }
 
-// One recursion step, in place: level j row mb from level j-1 row mb
+// sqrt(p/q) (prebuilt_arrays.jl:314-321) with constant p, q.
+template <typename T, int P, int Q>
+__device__ __forceinline__ T rootpq_c() {
+  return (T)__builtin_sqrt((double)P / (double)Q);
+}
+
+// One recursion step, in place: level JJ row MB from level JJ-1 row MB
 // (compute_ui.jl:71-100, :227-267).  Descending ma keeps the old values of
 // ma and ma-1 available.
-template <int J, typename S, int NK>
-__device__ __forceinline__ void wigner_step(const int j, const int mb, const CK<S>& c, S (&ur)[J + 1],
-                                            S (&ui)[J + 1], S (&dur)[NK > 0 ? NK : 1][J + 1],
-                                            S (&dui)[NK > 0 ? NK : 1][J + 1]) {
+template <int J, typename S, int NK, int JJ, int MB>
+__device__ __forceinline__ void wigner_step(const CK<S>& c, S (&ur)[J + 1], S (&ui)[J + 1],
+                                            S (&dur)[NK > 0 ? NK : 1][J + 1], S (&dui)[NK > 0 ? NK : 1][J + 1]) {
   using T = typename BaseOf<S>::type;
-#pragma unroll
-  for (int ma = J; ma >= 0; ma--) {
-    if (ma > j) continue;
+  static_for<0, JJ + 1>([&](auto ii) {
+    constexpr int ma = JJ - decltype(ii)::value;
     S nr = S((T)0), ni = S((T)0);
     S ndr[NK > 0 ? NK : 1], ndi[NK > 0 ? NK : 1];
 #pragma unroll
@@ -111,8 +175,8 @@ __device__ __forceinline__ void wigner_step(const int j, const int mb, const CK<
       ndr[k] = S((T)0);
       ndi[k] = S((T)0);
     }
-    if (ma >= 1) {
-      const T rb = rootpq<T>(ma, j - mb);
+    if constexpr (ma >= 1) {
+      const T rb = rootpq_c<T, ma, JJ - MB>();
       const S pr = ur[ma - 1], pi = ui[ma - 1];
       nr = -rb * (c.b_r * pr + c.b_i * pi);
       ni = -rb * (c.b_r * pi - c.b_i * pr);
@@ -122,8 +186,8 @@ __device__ __forceinline__ void wigner_step(const int j, const int mb, const CK<
         ndi[k] = -rb * (c.db_r[k] * pi - c.db_i[k] * pr + c.b_r * dui[k][ma - 1] - c.b_i * dur[k][ma - 1]);
       }
     }
-    if (ma <= j - 1) {
-      const T ra = rootpq<T>(j - ma, j - mb);
+    if constexpr (ma <= JJ - 1) {
+      const T ra = rootpq_c<T, JJ - ma, JJ - MB>();
       const S pr = ur[ma], pi = ui[ma];
       nr += ra * (c.a_r * pr + c.a_i * pi);
       ni += ra * (c.a_r * pi - c.a_i * pr);
@@ -140,24 +204,25 @@ __device__ __forceinline__ void wigner_step(const int j, const int mb, const CK<
       dur[k][ma] = ndr[k];
       dui[k][ma] = ndi[k];
     }
-  }
+  });
 }
 
 // Visits every unique left-half entry (level j, row mb, column ma with
 // 2mb < j, or 2mb == j and ma <= mb) once, calling
-//   emit(j, mb, ma, half, u_r, u_i, du_r[3], du_i[3])
+//   emit(IC<j>, IC<mb>, IC<ma>, IC<half>, IC<last>, u_r, u_i, du_r, du_i)
 // where half = 1 marks the centre of an even level (weight 1/2 in the
-// half-column sums of compute_b.jl) and du is the raw recursion derivative
-// (before the switching-function scaling of compute_ui.jl:300-325).
+// half-column sums of compute_b.jl), last = 1 marks the last entry of an
+// emitted row, and du_r/du_i are the raw recursion derivative rows (before
+// the switching-function scaling of compute_ui.jl:300-325).
 template <int J, typename S, int NK, typename Emit>
 __device__ __forceinline__ void wigner_chain(const CK<S>& c, Emit&& emit) {
   using T = typename BaseOf<S>::type;
   constexpr int NKA = NK > 0 ? NK : 1;
   S ur[J + 1], ui[J + 1], dur[NKA][J + 1], dui[NKA][J + 1];
   S sr[J + 1], si[J + 1], sdr[NKA][J + 1], sdi[NKA][J + 1];
-#pragma unroll
-  for (int mb = 0; 2 * mb <= J; mb++) {
-    if (mb == 0) {
+  static_for<0, J / 2 + 1>([&](auto mbc) {
+    constexpr int MB = decltype(mbc)::value;
+    if constexpr (MB == 0) {
       ur[0] = S((T)1);
       ui[0] = S((T)0);
 #pragma unroll
@@ -165,13 +230,13 @@ __device__ __forceinline__ void wigner_chain(const CK<S>& c, Emit&& emit) {
         dur[k][0] = S((T)0);
         dui[k][0] = S((T)0);
       }
-      emit(0, 0, 0, 1, ur[0], ui[0], dur, dui, 0);
+      emit(IC<0>{}, IC<0>{}, IC<0>{}, IC<1>{}, IC<1>{}, ur[0], ui[0], dur, dui);
     } else {
-      // row mb of level 2mb-1 = (-1)^(mb+x) conj(row mb-1 of level 2mb-1)[2mb-1-x]
-#pragma unroll
-      for (int x = 0; x < 2 * mb; x++) {
-        const bool neg = (mb + x) & 1;
-        const int y = 2 * mb - 1 - x;
+      // row MB of level 2MB-1 = (-1)^(MB+x) conj(row MB-1 of level 2MB-1)[2MB-1-x]
+      static_for<0, 2 * MB>([&](auto xc) {
+        constexpr int x = decltype(xc)::value;
+        constexpr bool neg = (MB + x) & 1;
+        constexpr int y = 2 * MB - 1 - x;
         ur[x] = neg ? -sr[y] : sr[y];
         ui[x] = neg ? si[y] : -si[y];
 #pragma unroll
@@ -179,13 +244,13 @@ __device__ __forceinline__ void wigner_chain(const CK<S>& c, Emit&& emit) {
           dur[k][x] = neg ? -sdr[k][y] : sdr[k][y];
           dui[k][x] = neg ? sdi[k][y] : -sdi[k][y];
         }
-      }
-      wigner_step<J, S, NK>(2 * mb, mb, c, ur, ui, dur, dui);
-      // middle row of the even level 2mb
-#pragma unroll
-      for (int ma = mb + 1; ma <= 2 * mb; ma++) {
-        const bool neg = (mb + ma) & 1;
-        const int y = 2 * mb - ma;
+      });
+      wigner_step<J, S, NK, 2 * MB, MB>(c, ur, ui, dur, dui);
+      // middle row of the even level 2MB
+      static_for<MB + 1, 2 * MB + 1>([&](auto mac) {
+        constexpr int ma = decltype(mac)::value;
+        constexpr bool neg = (MB + ma) & 1;
+        constexpr int y = 2 * MB - ma;
         ur[ma] = neg ? -ur[y] : ur[y];
         ui[ma] = neg ? ui[y] : -ui[y];
 #pragma unroll
@@ -193,21 +258,26 @@ __device__ __forceinline__ void wigner_chain(const CK<S>& c, Emit&& emit) {
           dur[k][ma] = neg ? -dur[k][y] : dur[k][y];
           dui[k][ma] = neg ? dui[k][y] : -dui[k][y];
         }
-      }
-      ui[mb] = -ui[mb];
+      });
+      ui[MB] = -ui[MB];
 #pragma unroll
-      for (int k = 0; k < NK; k++) dui[k][mb] = -dui[k][mb];
-#pragma unroll
-      for (int ma = 0; ma <= mb; ma++) emit(2 * mb, mb, ma, ma == mb ? 1 : 0, ur[ma], ui[ma], dur, dui, ma);
+      for (int k = 0; k < NK; k++) dui[k][MB] = -dui[k][MB];
+      static_for<0, MB + 1>([&](auto mac) {
+        constexpr int ma = decltype(mac)::value;
+        emit(IC<2 * MB>{}, IC<MB>{}, IC<ma>{}, IC<(ma == MB) ? 1 : 0>{}, IC<(ma == MB) ? 1 : 0>{}, ur[ma], ui[ma],
+             dur, dui);
+      });
     }
+    static_for<2 * MB + 1, J + 1>([&](auto jc) {
+      constexpr int JJ = decltype(jc)::value;
+      wigner_step<J, S, NK, JJ, MB>(c, ur, ui, dur, dui);
+      static_for<0, JJ + 1>([&](auto mac) {
+        constexpr int ma = decltype(mac)::value;
+        emit(IC<JJ>{}, IC<MB>{}, IC<ma>{}, IC<0>{}, IC<(ma == JJ) ? 1 : 0>{}, ur[ma], ui[ma], dur, dui);
+      });
+      if constexpr (JJ == 2 * MB + 1 && 2 * (MB + 1) <= J) {
 #pragma unroll
-    for (int j = 2 * mb + 1; j <= J; j++) {
-      wigner_step<J, S, NK>(j, mb, c, ur, ui, dur, dui);
-#pragma unroll
-      for (int ma = 0; ma <= j; ma++) emit(j, mb, ma, 0, ur[ma], ui[ma], dur, dui, ma);
-      if (j == 2 * mb + 1 && 2 * (mb + 1) <= J) {
-#pragma unroll
-        for (int x = 0; x <= j; x++) {
+        for (int x = 0; x <= JJ; x++) {
           sr[x] = ur[x];
           si[x] = ui[x];
 #pragma unroll
@@ -217,8 +287,8 @@ __device__ __forceinline__ void wigner_chain(const CK<S>& c, Emit&& emit) {
           }
         }
       }
-    }
-  }
+    });
+  });
 }
 
 // Pair geometry (+ tangent of r_ij = x_j - x_i along R.tang for Dual runs).
@@ -268,7 +338,8 @@ __global__ __launch_bounds__(64) void k_snap_ui(SnapDev P, SnapRun R, S* __restr
   __syncthreads();
   const int64_t q0 = valid ? R.off[a] : 0, q1 = valid ? R.off[a + 1] : 0;
   int nch = (int)((q1 - q0 + G - 1) / G);
-  for (int o = 1; o < 64; o <<= 1) nch = max(nch, __shfl_xor(nch, o, 64));
+  nch = max(max(__builtin_amdgcn_readlane(nch, 0), __builtin_amdgcn_readlane(nch, 16)),
+            max(__builtin_amdgcn_readlane(nch, 32), __builtin_amdgcn_readlane(nch, 48)));
   const int si = valid && R.species ? R.species[a] : 0;
   for (int ch = 0; ch < nch; ch++) {
     const int64_t q = q0 + (int64_t)ch * G + l;
@@ -287,16 +358,10 @@ __global__ __launch_bounds__(64) void k_snap_ui(SnapDev P, SnapRun R, S* __restr
       c.sfac = S((T)0);
     }
     const S w = c.sfac;
-    S dz[1][J + 1];
-    (void)dz;
-    wigner_chain<J, S, 0>(c, [&](int j, int mb, int ma, int, S u_r, S u_i, auto&, auto&, int) {
-      S vr = w * u_r, vi = w * u_i;
-#pragma unroll
-      for (int o = 1; o < G; o <<= 1) {
-        vr += shfl_xor_s(vr, o);
-        vi += shfl_xor_s(vi, o);
-      }
-      const int jju = ub_of(j) + (j + 1) * mb + ma;
+    wigner_chain<J, S, 0>(c, [&](auto jc, auto mbc, auto mac, auto, auto, S u_r, S u_i, auto&, auto&) {
+      constexpr int jju = ub_of(decltype(jc)::value) + (decltype(jc)::value + 1) * decltype(mbc)::value +
+                          decltype(mac)::value;
+      const S vr = group_sum<G>(w * u_r), vi = group_sum<G>(w * u_i);
       if (l == jju % G) {
         acc[g][jju][0] += vr;
         acc[g][jju][1] += vi;
@@ -341,6 +406,7 @@ __global__ __launch_bounds__(256) void k_snap_zby(SnapDev P, SnapRun R, const S*
                                                   S* __restrict__ Yout, S* __restrict__ Zout,
                                                   double* __restrict__ tanB) {
   using T = typename BaseOf<S>::type;
+  using A = typename AccOf<S>::type;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   S* U = reinterpret_cast<S*>(smem_raw);
   S* Z = U + 2 * P.idxu_max;
@@ -357,7 +423,7 @@ __global__ __launch_bounds__(256) void k_snap_zby(SnapDev P, SnapRun R, const S*
   for (int jjz = tid; jjz < P.idxz_max; jjz += 256) {
     const ZEnt ze = P.zent[jjz];
     const T* cg = cgt + ze.cgoff;
-    S zr = S((T)0), zi = S((T)0);
+    A zr = A(0.0), zi = A(0.0);
     int jju1 = P.idxu_block[ze.j1] + (ze.j1 + 1) * ze.mb1min;
     int jju2 = P.idxu_block[ze.j2] + (ze.j2 + 1) * ze.mb2max;
     int icgb = ze.mb1min * (ze.j2 + 1) + ze.mb2max;
@@ -376,18 +442,18 @@ __global__ __launch_bounds__(256) void k_snap_zby(SnapDev P, SnapRun R, const S*
         icga += ze.j2;
       }
       const T cgb = cg[icgb];
-      zr += cgb * sr;
-      zi += cgb * si_;
+      zr += to_acc(cgb * sr);
+      zi += to_acc(cgb * si_);
       jju1 += ze.j1 + 1;
       jju2 -= ze.j2 + 1;
       icgb += ze.j2;
     }
     if (P.bnorm) {
-      zr = zr / (T)(ze.j + 1);
-      zi = zi / (T)(ze.j + 1);
+      zr = zr / (double)(ze.j + 1);
+      zi = zi / (double)(ze.j + 1);
     }
-    Z[2 * jjz] = zr;
-    Z[2 * jjz + 1] = zi;
+    Z[2 * jjz] = from_acc<S>(zr);
+    Z[2 * jjz + 1] = from_acc<S>(zi);
   }
   __syncthreads();
   if (Zout) {
@@ -404,16 +470,18 @@ __global__ __launch_bounds__(256) void k_snap_zby(SnapDev P, SnapRun R, const S*
     const BTerm bt = P.bterm[jjb];
     const int j = bt.j;
     int jju = P.idxu_block[j], jjz = bt.jjz0[0];
-    S sum = S((T)0);
+    A sum = A(0.0);
     for (int mb = 0; 2 * mb < j; mb++)
-      for (int ma = 0; ma <= j; ma++, jju++, jjz++) sum += U[2 * jju] * Z[2 * jjz] + U[2 * jju + 1] * Z[2 * jjz + 1];
+      for (int ma = 0; ma <= j; ma++, jju++, jjz++)
+        sum += to_acc(U[2 * jju] * Z[2 * jjz] + U[2 * jju + 1] * Z[2 * jjz + 1]);
     if (j % 2 == 0) {
       const int mb = j / 2;
-      for (int ma = 0; ma < mb; ma++, jju++, jjz++) sum += U[2 * jju] * Z[2 * jjz] + U[2 * jju + 1] * Z[2 * jjz + 1];
-      sum += (T)0.5 * (U[2 * jju] * Z[2 * jjz] + U[2 * jju + 1] * Z[2 * jjz + 1]);
+      for (int ma = 0; ma < mb; ma++, jju++, jjz++)
+        sum += to_acc(U[2 * jju] * Z[2 * jjz] + U[2 * jju + 1] * Z[2 * jjz + 1]);
+      sum += 0.5 * to_acc(U[2 * jju] * Z[2 * jjz] + U[2 * jju + 1] * Z[2 * jjz + 1]);
     }
-    S b = (T)2 * sum;
-    if (P.bzero) b = b - (T)P.bzero_j[j];
+    A b = 2.0 * sum;
+    if (P.bzero) b = b - P.bzero_j[j];
     const double bv = value_of(b);
     if (Bout) Bout[a * P.ncoeff + jjb] = bv;
     if (beta) epart += beta[s * P.ncoeff + jjb] * bv;
@@ -429,16 +497,16 @@ __global__ __launch_bounds__(256) void k_snap_zby(SnapDev P, SnapRun R, const S*
     S* yg = Yout + a * (int64_t)P.idxu_max * 2;
     const T* yc = ycoef + (int64_t)s * P.nblk;
     for (int h = tid; h < P.nhalf; h += 256) {
-      S yr = S((T)0), yi = S((T)0);
+      A yr = A(0.0), yi = A(0.0);
       for (int k = P.yoff[h]; k < P.yoff[h + 1]; k++) {
         const int jjz = P.yjjz[k];
         const T cf = yc[P.zent[jjz].blk];
-        yr += cf * Z[2 * jjz];
-        yi += cf * Z[2 * jjz + 1];
+        yr += to_acc(cf * Z[2 * jjz]);
+        yi += to_acc(cf * Z[2 * jjz + 1]);
       }
       const int jju = P.half_jju[h];
-      yg[2 * jju] = yr;
-      yg[2 * jju + 1] = yi;
+      yg[2 * jju] = from_acc<S>(yr);
+      yg[2 * jju + 1] = from_acc<S>(yi);
     }
   }
 }
@@ -463,23 +531,35 @@ __global__ __launch_bounds__(256) void k_snap_dedr(SnapDev P, SnapRun R, const S
   CK<S> c;
   cayley_klein<S, true>(x, y, z, rcut, P.weight[sj], P, c);
   const S* Yi = Y + (int64_t)i * IDXU * 2;
+  // per-row partial sums in S, flushed into A (FP64 for FP32 runs) at the
+  // end of every emitted row
+  using A = typename AccOf<S>::type;
+  A a0 = A(0.0), a1 = A(0.0), a2 = A(0.0), a3 = A(0.0);
   S s0 = S((T)0), s1 = S((T)0), s2 = S((T)0), s3 = S((T)0);
-  wigner_chain<J, S, 3>(c, [&](int jl, int mb, int ma, int half, S u_r, S u_i, auto& dur, auto& dui, int col) {
-    const int jju = ub_of(jl) + (jl + 1) * mb + ma;
+  wigner_chain<J, S, 3>(c, [&](auto jc, auto mbc, auto mac, auto hc, auto lc, S u_r, S u_i, auto& dur, auto& dui) {
+    constexpr int jl = decltype(jc)::value, mb = decltype(mbc)::value, ma = decltype(mac)::value;
+    constexpr int jju = ub_of(jl) + (jl + 1) * mb + ma;
     S yr = Yi[2 * jju], yi = Yi[2 * jju + 1];
-    if (half) {
+    if constexpr (decltype(hc)::value == 1) {
       yr = (T)0.5 * yr;
       yi = (T)0.5 * yi;
     }
     s0 += u_r * yr + u_i * yi;
-    s1 += dur[0][col] * yr + dui[0][col] * yi;
-    s2 += dur[1][col] * yr + dui[1][col] * yi;
-    s3 += dur[2][col] * yr + dui[2][col] * yi;
+    s1 += dur[0][ma] * yr + dui[0][ma] * yi;
+    s2 += dur[1][ma] * yr + dui[1][ma] * yi;
+    s3 += dur[2][ma] * yr + dui[2][ma] * yi;
+    if constexpr (decltype(lc)::value == 1) {
+      a0 += to_acc(s0);
+      a1 += to_acc(s1);
+      a2 += to_acc(s2);
+      a3 += to_acc(s3);
+      s0 = s1 = s2 = s3 = S((T)0);
+    }
   });
-  const S g0 = c.dsfac * s0;
-  S d0 = (T)2 * (g0 * c.u[0] + c.sfac * s1);
-  S d1 = (T)2 * (g0 * c.u[1] + c.sfac * s2);
-  S d2 = (T)2 * (g0 * c.u[2] + c.sfac * s3);
+  const A g0 = to_acc(c.dsfac) * a0, sf = to_acc(c.sfac);
+  const A d0 = 2.0 * (g0 * to_acc(c.u[0]) + sf * a1);
+  const A d1 = 2.0 * (g0 * to_acc(c.u[1]) + sf * a2);
+  const A d2 = 2.0 * (g0 * to_acc(c.u[2]) + sf * a3);
   dedr[3 * q] = value_of(d0);
   dedr[3 * q + 1] = value_of(d1);
   dedr[3 * q + 2] = value_of(d2);
