@@ -32,13 +32,16 @@ void* scratch(ipg_ctx* ctx, const char* name, size_t bytes) {
 }
 
 hipEvent_t get_event(ipg_ctx* ctx) {
-  if (!ctx->free_events.empty()) {
-    hipEvent_t e = ctx->free_events.back();
-    ctx->free_events.pop_back();
-    return e;
+  if (ctx->free_events.empty()) {
+    // create in batches: event creation is far slower than a launch
+    for (int k = 0; k < 256; k++) {
+      hipEvent_t e;
+      IPG_HIP(hipEventCreate(&e));
+      ctx->free_events.push_back(e);
+    }
   }
-  hipEvent_t e;
-  IPG_HIP(hipEventCreate(&e));
+  hipEvent_t e = ctx->free_events.back();
+  ctx->free_events.pop_back();
   return e;
 }
 

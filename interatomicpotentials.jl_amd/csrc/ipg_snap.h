@@ -49,6 +49,8 @@ struct SnapHost {
   std::vector<int> half_jju;                   // nhalf left-half positions (mb <= j/2)
   std::vector<int> yoff;                       // nhalf+1
   std::vector<int> yjjz;                       // jjz list per left-half position
+  std::vector<int> yjblk;                      // Z block of each yjjz entry
+  std::vector<uint32_t> zplan;                 // compact Z loop plan, 3 words per entry
 };
 
 void snap_build_host(SnapHost& h, const ipg_snap_params* p);
@@ -70,6 +72,9 @@ struct SnapDev {
   const int* half_jju;
   const int* yoff;
   const int* yjjz;
+  const int* yjblk;        // Z block of each yjjz entry
+  const uint32_t* zplan;   // 3 words per Z entry, see snap_build_host
+  int idxcg_max;
 };
 
 // Per-call device data of one SNAP evaluation.

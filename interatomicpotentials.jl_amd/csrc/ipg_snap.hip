@@ -172,6 +172,23 @@ void snap_build_host(SnapHost& h, const ipg_snap_params* p) {
         h.yoff.push_back((int)h.yjjz.size());
       }
   h.nhalf = (int)h.half_jju.size();
+  h.yjblk.clear();
+  for (int k : h.yjjz) h.yjblk.push_back(h.zent[k].blk);
+  // compact Z loop plans (see k_snap_zby): u1/u2 start, absolute CG offsets,
+  // na, nb, row strides and j+1
+  h.zplan.clear();
+  for (const ZEnt& z : h.zent) {
+    const uint32_t u1b = h.idxu_block[z.j1] + (z.j1 + 1) * z.mb1min + z.ma1min;
+    const uint32_t u2b = h.idxu_block[z.j2] + (z.j2 + 1) * z.mb2max + z.ma2max;
+    const uint32_t cga0 = z.cgoff + z.ma1min * (z.j2 + 1) + z.ma2max;
+    const uint32_t cgb0 = z.cgoff + z.mb1min * (z.j2 + 1) + z.mb2max;
+    IPG_REQUIRE(u1b < 65536 && u2b < 65536 && cga0 < 65536 && cgb0 < 65536 && z.na < 16 && z.nb < 16,
+                "internal: Z plan field overflow");
+    h.zplan.push_back(u1b | (u2b << 16));
+    h.zplan.push_back(cga0 | (cgb0 << 16));
+    h.zplan.push_back((uint32_t)z.na | ((uint32_t)z.nb << 4) | ((uint32_t)(z.j1 + 1) << 8) | ((uint32_t)z.j2 << 12) |
+                      ((uint32_t)(z.j + 1) << 16));
+  }
   h.wself = 1.0;
   h.bzero_j.assign(jd, 0.0);
   if (h.bzero)
@@ -220,6 +237,8 @@ static void snap_upload(ipg_snap* sn) {
   size_t o_hj = place(sizeof(int) * h.half_jju.size());
   size_t o_yo = place(sizeof(int) * h.yoff.size());
   size_t o_yj = place(sizeof(int) * std::max<size_t>(1, h.yjjz.size()));
+  size_t o_yb = place(sizeof(int) * std::max<size_t>(1, h.yjblk.size()));
+  size_t o_zp = place(sizeof(uint32_t) * std::max<size_t>(1, h.zplan.size()));
   if (hipMalloc(&sn->dmem, off) != hipSuccess) {
     (void)hipGetLastError();
     sn->dmem = nullptr;
@@ -235,6 +254,8 @@ static void snap_upload(ipg_snap* sn) {
   h2d(ctx, base + o_hj, h.half_jju.data(), sizeof(int) * h.half_jju.size());
   h2d(ctx, base + o_yo, h.yoff.data(), sizeof(int) * h.yoff.size());
   if (!h.yjjz.empty()) h2d(ctx, base + o_yj, h.yjjz.data(), sizeof(int) * h.yjjz.size());
+  if (!h.yjblk.empty()) h2d(ctx, base + o_yb, h.yjblk.data(), sizeof(int) * h.yjblk.size());
+  h2d(ctx, base + o_zp, h.zplan.data(), sizeof(uint32_t) * h.zplan.size());
   sync(ctx);
   d.zent = (const ZEnt*)(base + o_z);
   d.yblk = (const YBlk*)(base + o_y);
@@ -244,6 +265,9 @@ static void snap_upload(ipg_snap* sn) {
   d.half_jju = (const int*)(base + o_hj);
   d.yoff = (const int*)(base + o_yo);
   d.yjjz = (const int*)(base + o_yj);
+  d.yjblk = (const int*)(base + o_yb);
+  d.zplan = (const uint32_t*)(base + o_zp);
+  d.idxcg_max = h.idxcg_max;
 }
 
 // ---------------------------------------------------------------------------

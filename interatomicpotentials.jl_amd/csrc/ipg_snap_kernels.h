@@ -15,6 +15,8 @@
 #pragma once
 #include "ipg_snap.h"
 
+#include <algorithm>
+
 namespace ipg {
 
 __host__ __device__ constexpr int ub_of(int j) { return j * (j + 1) * (2 * j + 1) / 6; }  // idxu_block[j]
@@ -92,7 +94,7 @@ struct CK {
 // Cayley-Klein parameters of one neighbor and their gradients
 // (compute_ui.jl:328-347 compute_duidrj, :191-216; sfac/dsfac :350-380).
 template <typename S, bool DERIV>
-__device__ __forceinline__ void cayley_klein(S x, S y, S z, double rcut, double wj, const SnapDev& P, CK<S>& c) {
+__device__ __forceinline__ void cayley_klein_impl(S x, S y, S z, double rcut, double wj, const SnapDev& P, CK<S>& c) {
   using T = typename BaseOf<S>::type;
   S rsq = x * x + y * y + z * z;
   S r = sqrt(rsq);
@@ -136,6 +138,41 @@ __device__ __forceinline__ void cayley_klein(S x, S y, S z, double rcut, double 
     c.da_i[2] = c.da_i[2] - r0inv;
     c.db_i[0] = c.db_i[0] - r0inv;
     c.db_r[1] = c.db_r[1] + r0inv;
+  }
+}
+
+// The Cayley-Klein parameters are evaluated at FP64 width (Dual<double> for
+// tangent runs) and rounded once to the kernel's scalar type: cot(theta0)
+// near the cutoff amplifies FP32 rounding by ~1/sin^2(theta0).
+template <typename S> struct WideOf { using type = double; };
+template <typename T> struct WideOf<Dual<T>> { using type = Dual<double>; };
+template <typename S> __device__ __forceinline__ S narrow(double x) { return (S)x; }
+template <typename S> __device__ __forceinline__ S narrow(Dual<double> x) {
+  using T = typename BaseOf<S>::type;
+  return S((T)x.v, (T)x.t);
+}
+template <typename S, bool DERIV>
+__device__ __forceinline__ void cayley_klein(typename WideOf<S>::type x, typename WideOf<S>::type y,
+                                             typename WideOf<S>::type z, double rcut, double wj, const SnapDev& P,
+                                             CK<S>& c) {
+  using W = typename WideOf<S>::type;
+  CK<W> w;
+  cayley_klein_impl<W, DERIV>(x, y, z, rcut, wj, P, w);
+  c.a_r = narrow<S>(w.a_r);
+  c.a_i = narrow<S>(w.a_i);
+  c.b_r = narrow<S>(w.b_r);
+  c.b_i = narrow<S>(w.b_i);
+  c.sfac = narrow<S>(w.sfac);
+  c.dsfac = narrow<S>(w.dsfac);
+  if (DERIV) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      c.da_r[k] = narrow<S>(w.da_r[k]);
+      c.da_i[k] = narrow<S>(w.da_i[k]);
+      c.db_r[k] = narrow<S>(w.db_r[k]);
+      c.db_i[k] = narrow<S>(w.db_i[k]);
+      c.u[k] = narrow<S>(w.u[k]);
+    }
   }
 }
 
@@ -348,8 +385,8 @@ __global__ __launch_bounds__(64) void k_snap_ui(SnapDev P, SnapRun R, S* __restr
     if (act) {
       const int32_t j = R.nbr[q];
       const int sj = R.species ? R.species[j] : 0;
-      S x, y, z;
-      load_pair<S>(R, q, (int32_t)a, j, x, y, z);
+      typename WideOf<S>::type x, y, z;
+      load_pair<typename WideOf<S>::type>(R, q, (int32_t)a, j, x, y, z);
       const double rcut = (P.radii[si] + P.radii[sj]) * P.rcutfac;
       cayley_klein<S, false>(x, y, z, rcut, P.weight[sj], P, c);
     } else {
@@ -394,119 +431,142 @@ __global__ __launch_bounds__(64) void k_snap_ui(SnapDev P, SnapRun R, S* __restr
 
 // ---------------------------------------------------------------------------
 // K2: Z (compute_z.jl:1-62), B (compute_b.jl:67-157) and the adjoint
-// Y = dE/dU for one atom per 256-thread block.  U and Z live in LDS.
+// Y = dE/dU.  Persistent: each 256-thread block stages the Clebsch-Gordan
+// table in LDS once, keeps its Z entries' compact plans in registers and
+// loops over atoms; U and Z of the current atom live in LDS.
 // Y_l(e) = sum over Z blocks (a,c,l) of coef(a,c,l) Z_(a,c,l)(e), with coef
 // collecting the beta entries whose dB/dr half-sums use that block.
 // ---------------------------------------------------------------------------
+constexpr int ZBY_THREADS = 256;
+constexpr int ZBY_KMAX = 10;  // ceil(idxz_max / 256) at 2J = 8
+
 template <typename S>
-__global__ __launch_bounds__(256) void k_snap_zby(SnapDev P, SnapRun R, const S* __restrict__ ulisttot,
-                                                  const double* __restrict__ beta,
-                                                  const typename BaseOf<S>::type* __restrict__ ycoef,
-                                                  double* __restrict__ Bout, double* __restrict__ eatom,
-                                                  S* __restrict__ Yout, S* __restrict__ Zout,
-                                                  double* __restrict__ tanB) {
+__global__ __launch_bounds__(ZBY_THREADS) void k_snap_zby(SnapDev P, SnapRun R, const S* __restrict__ ulisttot,
+                                                          const double* __restrict__ beta,
+                                                          const typename BaseOf<S>::type* __restrict__ ycoef,
+                                                          double* __restrict__ Bout, double* __restrict__ eatom,
+                                                          S* __restrict__ Yout, S* __restrict__ Zout,
+                                                          double* __restrict__ tanB) {
   using T = typename BaseOf<S>::type;
   using A = typename AccOf<S>::type;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  S* U = reinterpret_cast<S*>(smem_raw);
+  T* cgL = reinterpret_cast<T*>(smem_raw);
+  S* U = reinterpret_cast<S*>(smem_raw + (((size_t)P.idxcg_max * sizeof(T) + 15) & ~(size_t)15));
   S* Z = U + 2 * P.idxu_max;
-  __shared__ double red[4];
+  __shared__ double red[ZBY_THREADS / 64];
   const int tid = threadIdx.x;
-  const int64_t a = blockIdx.x;
-  const int s = R.species ? R.species[a] : 0;
-  const S* ug = ulisttot + a * (int64_t)P.idxu_max * 2;
-  for (int e = tid; e < 2 * P.idxu_max; e += 256) U[e] = ug[e];
-  __syncthreads();
   const T* cgt;
   if constexpr (std::is_same<T, float>::value) cgt = P.cg_f;
   else cgt = P.cg_d;
-  for (int jjz = tid; jjz < P.idxz_max; jjz += 256) {
-    const ZEnt ze = P.zent[jjz];
-    const T* cg = cgt + ze.cgoff;
-    A zr = A(0.0), zi = A(0.0);
-    int jju1 = P.idxu_block[ze.j1] + (ze.j1 + 1) * ze.mb1min;
-    int jju2 = P.idxu_block[ze.j2] + (ze.j2 + 1) * ze.mb2max;
-    int icgb = ze.mb1min * (ze.j2 + 1) + ze.mb2max;
-    for (int ib = 0; ib < ze.nb; ib++) {
-      S sr = S((T)0), si_ = S((T)0);
-      int ma1 = ze.ma1min, ma2 = ze.ma2max;
-      int icga = ze.ma1min * (ze.j2 + 1) + ze.ma2max;
-      for (int ia = 0; ia < ze.na; ia++) {
-        const S u1r = U[2 * (jju1 + ma1)], u1i = U[2 * (jju1 + ma1) + 1];
-        const S u2r = U[2 * (jju2 + ma2)], u2i = U[2 * (jju2 + ma2) + 1];
-        const T cga = cg[icga];
-        sr += cga * (u1r * u2r - u1i * u2i);
-        si_ += cga * (u1r * u2i + u1i * u2r);
-        ma1++;
-        ma2--;
-        icga += ze.j2;
-      }
-      const T cgb = cg[icgb];
-      zr += to_acc(cgb * sr);
-      zi += to_acc(cgb * si_);
-      jju1 += ze.j1 + 1;
-      jju2 -= ze.j2 + 1;
-      icgb += ze.j2;
+  for (int e = tid; e < P.idxcg_max; e += ZBY_THREADS) cgL[e] = cgt[e];
+  uint32_t zp0[ZBY_KMAX], zp1[ZBY_KMAX], zp2[ZBY_KMAX];
+#pragma unroll
+  for (int k = 0; k < ZBY_KMAX; k++) {
+    const int jjz = tid + ZBY_THREADS * k;
+    zp0[k] = zp1[k] = zp2[k] = 0;
+    if (jjz < P.idxz_max) {
+      zp0[k] = P.zplan[3 * jjz];
+      zp1[k] = P.zplan[3 * jjz + 1];
+      zp2[k] = P.zplan[3 * jjz + 2];
     }
-    if (P.bnorm) {
-      zr = zr / (double)(ze.j + 1);
-      zi = zi / (double)(ze.j + 1);
-    }
-    Z[2 * jjz] = from_acc<S>(zr);
-    Z[2 * jjz + 1] = from_acc<S>(zi);
   }
-  __syncthreads();
-  if (Zout) {
-    S* zg = Zout + a * (int64_t)P.idxz_max * 2;
-    for (int e = tid; e < 2 * P.idxz_max; e += 256) zg[e] = Z[e];
-  }
-  // B and the per-atom energy beta_s . B
-  double epart = 0.0;
   const int M = P.ncoeff * P.nspecies;
-  if (tanB)
-    for (int c = tid; c < M; c += 256) tanB[a * M + c] = 0.0;
-  if (tanB) __syncthreads();
-  for (int jjb = tid; jjb < P.idxb_max; jjb += 256) {
-    const BTerm bt = P.bterm[jjb];
-    const int j = bt.j;
-    int jju = P.idxu_block[j], jjz = bt.jjz0[0];
-    A sum = A(0.0);
-    for (int mb = 0; 2 * mb < j; mb++)
-      for (int ma = 0; ma <= j; ma++, jju++, jjz++)
-        sum += to_acc(U[2 * jju] * Z[2 * jjz] + U[2 * jju + 1] * Z[2 * jjz + 1]);
-    if (j % 2 == 0) {
-      const int mb = j / 2;
-      for (int ma = 0; ma < mb; ma++, jju++, jjz++)
-        sum += to_acc(U[2 * jju] * Z[2 * jjz] + U[2 * jju + 1] * Z[2 * jjz + 1]);
-      sum += 0.5 * to_acc(U[2 * jju] * Z[2 * jjz] + U[2 * jju + 1] * Z[2 * jjz + 1]);
-    }
-    A b = 2.0 * sum;
-    if (P.bzero) b = b - P.bzero_j[j];
-    const double bv = value_of(b);
-    if (Bout) Bout[a * P.ncoeff + jjb] = bv;
-    if (beta) epart += beta[s * P.ncoeff + jjb] * bv;
-    if (tanB) tanB[a * M + s * P.ncoeff + jjb] = tangent_of(b);
-  }
-  if (eatom) {
-    for (int o = 32; o > 0; o >>= 1) epart += __shfl_xor(epart, o, 64);
-    if ((tid & 63) == 0) red[tid >> 6] = epart;
+  for (int64_t a = blockIdx.x; a < R.n; a += gridDim.x) {
+    __syncthreads();  // the previous atom's readers of U and Z are done
+    const int s = R.species ? R.species[a] : 0;
+    const S* ug = ulisttot + a * (int64_t)P.idxu_max * 2;
+    for (int e = tid; e < 2 * P.idxu_max; e += ZBY_THREADS) U[e] = ug[e];
     __syncthreads();
-    if (tid == 0) eatom[a] = ((red[0] + red[1]) + red[2]) + red[3];
-  }
-  if (Yout) {
-    S* yg = Yout + a * (int64_t)P.idxu_max * 2;
-    const T* yc = ycoef + (int64_t)s * P.nblk;
-    for (int h = tid; h < P.nhalf; h += 256) {
-      A yr = A(0.0), yi = A(0.0);
-      for (int k = P.yoff[h]; k < P.yoff[h + 1]; k++) {
-        const int jjz = P.yjjz[k];
-        const T cf = yc[P.zent[jjz].blk];
-        yr += to_acc(cf * Z[2 * jjz]);
-        yi += to_acc(cf * Z[2 * jjz + 1]);
+#pragma unroll
+    for (int k = 0; k < ZBY_KMAX; k++) {
+      const int jjz = tid + ZBY_THREADS * k;
+      if (jjz < P.idxz_max) {
+        // plan: u1 base | u2 base << 16; cga0 | cgb0 << 16; na | nb<<4 | (j1+1)<<8 | j2<<12 | (j+1)<<16
+        const int u1b = (int)(zp0[k] & 0xFFFF), u2b = (int)(zp0[k] >> 16);
+        const int cga0 = (int)(zp1[k] & 0xFFFF), cgb0 = (int)(zp1[k] >> 16);
+        const int na = (int)(zp2[k] & 0xF), nb = (int)((zp2[k] >> 4) & 0xF);
+        const int s1 = (int)((zp2[k] >> 8) & 0xF), j2 = (int)((zp2[k] >> 12) & 0xF);
+        A zr = A(0.0), zi = A(0.0);
+        int u1 = u1b, u2 = u2b, cgb = cgb0;
+        for (int ib = 0; ib < nb; ib++) {
+          S sr = S((T)0), si_ = S((T)0);
+          int cga = cga0;
+          for (int ia = 0; ia < na; ia++) {
+            const S u1r = U[2 * (u1 + ia)], u1i = U[2 * (u1 + ia) + 1];
+            const S u2r = U[2 * (u2 - ia)], u2i = U[2 * (u2 - ia) + 1];
+            const T c = cgL[cga];
+            sr += c * (u1r * u2r - u1i * u2i);
+            si_ += c * (u1r * u2i + u1i * u2r);
+            cga += j2;
+          }
+          const T cb = cgL[cgb];
+          zr += to_acc(cb * sr);
+          zi += to_acc(cb * si_);
+          u1 += s1;
+          u2 -= j2 + 1;
+          cgb += j2;
+        }
+        if (P.bnorm) {
+          const double jp1 = (double)(zp2[k] >> 16);
+          zr = zr / jp1;
+          zi = zi / jp1;
+        }
+        Z[2 * jjz] = from_acc<S>(zr);
+        Z[2 * jjz + 1] = from_acc<S>(zi);
       }
-      const int jju = P.half_jju[h];
-      yg[2 * jju] = from_acc<S>(yr);
-      yg[2 * jju + 1] = from_acc<S>(yi);
+    }
+    __syncthreads();
+    if (Zout) {
+      S* zg = Zout + a * (int64_t)P.idxz_max * 2;
+      for (int e = tid; e < 2 * P.idxz_max; e += ZBY_THREADS) zg[e] = Z[e];
+    }
+    // B and the per-atom energy beta_s . B
+    double epart = 0.0;
+    if (tanB)
+      for (int c = tid; c < M; c += ZBY_THREADS)
+        if (c / P.ncoeff != s) tanB[a * M + c] = 0.0;
+    for (int jjb = tid; jjb < P.idxb_max; jjb += ZBY_THREADS) {
+      const BTerm bt = P.bterm[jjb];
+      const int j = bt.j;
+      int jju = P.idxu_block[j], jjz = bt.jjz0[0];
+      A sum = A(0.0);
+      for (int mb = 0; 2 * mb < j; mb++)
+        for (int ma = 0; ma <= j; ma++, jju++, jjz++)
+          sum += to_acc(U[2 * jju] * Z[2 * jjz] + U[2 * jju + 1] * Z[2 * jjz + 1]);
+      if (j % 2 == 0) {
+        const int mb = j / 2;
+        for (int ma = 0; ma < mb; ma++, jju++, jjz++)
+          sum += to_acc(U[2 * jju] * Z[2 * jjz] + U[2 * jju + 1] * Z[2 * jjz + 1]);
+        sum += 0.5 * to_acc(U[2 * jju] * Z[2 * jjz] + U[2 * jju + 1] * Z[2 * jjz + 1]);
+      }
+      A b = 2.0 * sum;
+      if (P.bzero) b = b - P.bzero_j[j];
+      const double bv = value_of(b);
+      if (Bout) Bout[a * P.ncoeff + jjb] = bv;
+      if (beta) epart += beta[s * P.ncoeff + jjb] * bv;
+      if (tanB) tanB[a * M + s * P.ncoeff + jjb] = tangent_of(b);
+    }
+    if (eatom) {
+      for (int o = 32; o > 0; o >>= 1) epart += __shfl_xor(epart, o, 64);
+      if ((tid & 63) == 0) red[tid >> 6] = epart;
+      __syncthreads();
+      if (tid == 0) eatom[a] = ((red[0] + red[1]) + red[2]) + red[3];
+    }
+    if (Yout) {
+      S* yg = Yout + a * (int64_t)P.idxu_max * 2;
+      const T* yc = ycoef + (int64_t)s * P.nblk;
+      for (int h = tid; h < P.nhalf; h += ZBY_THREADS) {
+        A yr = A(0.0), yi = A(0.0);
+        for (int k = P.yoff[h]; k < P.yoff[h + 1]; k++) {
+          const int jjz = P.yjjz[k];
+          const T cf = yc[P.yjblk[k]];
+          yr += to_acc(cf * Z[2 * jjz]);
+          yi += to_acc(cf * Z[2 * jjz + 1]);
+        }
+        const int jju = P.half_jju[h];
+        yg[2 * jju] = from_acc<S>(yr);
+        yg[2 * jju + 1] = from_acc<S>(yi);
+      }
     }
   }
 }
@@ -525,8 +585,8 @@ __global__ __launch_bounds__(256) void k_snap_dedr(SnapDev P, SnapRun R, const S
   if (q >= R.npairs) return;
   const int32_t i = R.center[q], j = R.nbr[q];
   const int si = R.species ? R.species[i] : 0, sj = R.species ? R.species[j] : 0;
-  S x, y, z;
-  load_pair<S>(R, q, i, j, x, y, z);
+  typename WideOf<S>::type x, y, z;
+  load_pair<typename WideOf<S>::type>(R, q, i, j, x, y, z);
   const double rcut = (P.radii[si] + P.radii[sj]) * P.rcutfac;
   CK<S> c;
   cayley_klein<S, true>(x, y, z, rcut, P.weight[sj], P, c);
@@ -598,7 +658,7 @@ template <typename S>
 void snap_launch_ui(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, S* ulisttot) {
   if (R.n == 0) return;
   const double avg = (double)R.npairs / (double)R.n;
-  const int apw = avg <= 24.0 ? 2 : 1;
+  const int apw = avg <= 30.0 ? 2 : 1;  // 32 lanes per atom hold up to 32 neighbors in one chunk
   const char* nm = IsDual<S>::value ? "snap_ui_dual" : (sizeof(S) == 4 ? "snap_ui_fp32" : "snap_ui_fp64");
   launch(ctx, nm, [&] {
     switch (P.J) {
@@ -619,14 +679,19 @@ template <typename S>
 void snap_launch_zby(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const S* ulisttot, const double* beta,
                      const typename BaseOf<S>::type* ycoef, double* Bout, double* eatom, S* Yout, S* Zout,
                      double* tanB) {
+  using T = typename BaseOf<S>::type;
   if (R.n == 0) return;
-  size_t sm = sizeof(S) * 2 * (size_t)(P.idxu_max + P.idxz_max);
+  IPG_REQUIRE(P.idxz_max <= ZBY_THREADS * ZBY_KMAX, "internal: Z table larger than the kernel plan");
+  const size_t sm = (((size_t)P.idxcg_max * sizeof(T) + 15) & ~(size_t)15) +
+                    sizeof(S) * 2 * (size_t)(P.idxu_max + P.idxz_max);
+  const int per_cu = std::max(1, std::min(8, (int)((160 * 1024) / (sm + 1024))));
+  const int64_t grid = std::min<int64_t>(R.n, (int64_t)ctx->num_cus * per_cu);
   const char* nm = IsDual<S>::value ? "snap_zby_dual" : (sizeof(S) == 4 ? "snap_zby_fp32" : "snap_zby_fp64");
   launch(ctx, nm, [&] {
     if (sm > 64 * 1024)
       IPG_HIP(hipFuncSetAttribute((const void*)k_snap_zby<S>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
-    k_snap_zby<S><<<(unsigned)R.n, 256, sm, ctx->stream>>>(P, R, ulisttot, beta, ycoef, Bout, eatom, Yout, Zout,
-                                                           tanB);
+    k_snap_zby<S><<<(unsigned)grid, ZBY_THREADS, sm, ctx->stream>>>(P, R, ulisttot, beta, ycoef, Bout, eatom, Yout,
+                                                                    Zout, tanB);
   });
 }
 
