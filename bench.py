@@ -186,7 +186,7 @@ def main():
     import ipgpu
     from ipgpu import perfmodel
     from ipgpu._lib import IPG_SNAP_ADJOINT_FP32, IPG_SNAP_PLAIN_FP64, IPG_SNAP_ADJOINT_FP64
-    from ipgpu.device import prof_enable, prof_reset
+    from ipgpu.device import prof_enable, prof_reset, set_graphs
 
     ipgpu.load()
     log("setup SNAP workload")
@@ -198,12 +198,19 @@ def main():
     prof_enable(True)
     t32 = time_steps(lambda: snap_step(sn, dsys, d, IPG_SNAP_ADJOINT_FP32), a.steps, a.warmup)
     prof_enable(False)
-    k32 = kernel_avg(["snap_ui_fp32", "snap_zby_fp32", "snap_dedr_fp32", "snap_force_gather", "snap_list_count",
+    k32 = kernel_avg(["snap_ui_fp32", "snap_zy_fp32", "snap_dedr_fp32", "snap_force_gather", "snap_list_count",
                       "snap_list_fill", "nlist_count", "nlist_fill", "snap_ycoef", "reduce_rows", "scan",
                       "rev_sort", "cell_sort"])
     f32 = d["f"].download()
     e32 = float(d["e"].download()[0])
-    log("fp32 adjoint: %.3f ms/step" % (t32 * 1e3))
+    ginfo = sn.graph_info()
+    log("fp32 adjoint: %.3f ms/step (graph %s)" % (t32 * 1e3, ginfo))
+    # the same step without profiling events, and without graph replay
+    t32_noev = time_steps(lambda: snap_step(sn, dsys, d, IPG_SNAP_ADJOINT_FP32), a.steps, 1)
+    set_graphs(False)
+    t32_eager = time_steps(lambda: snap_step(sn, dsys, d, IPG_SNAP_ADJOINT_FP32), a.steps, 1)
+    set_graphs(True)
+    log("fp32 adjoint without events %.3f ms, uncaptured %.3f ms" % (t32_noev * 1e3, t32_eager * 1e3))
     # FP64 plain (the reference's route) and FP64 adjoint
     steps64 = max(3, a.steps // 4)
     t64 = time_steps(lambda: snap_step(sn, dsys, d, IPG_SNAP_PLAIN_FP64), steps64, 1)
@@ -218,7 +225,7 @@ def main():
     npairs = len(nl.j)
     flops = {
         "snap_ui_fp32": perfmodel.flops_ui_per_pair(8) * npairs,
-        "snap_zby_fp32": perfmodel.flops_zby_per_atom(8) * n,
+        "snap_zy_fp32": perfmodel.flops_zby_per_atom(8) * n,
         "snap_dedr_fp32": perfmodel.flops_dedr_per_pair(8) * npairs,
     }
     dom = max((k for k in flops if k in k32), key=lambda k: k32[k][1])
@@ -256,6 +263,9 @@ def main():
             "fp64_adjoint_atoms_per_s": n / t64a,
             "speedup_fp32_adjoint_vs_fp64_plain": t64 / t32,
             "kernels_fp32_ms": {k: v[1] for k, v in k32.items()},
+            "fp32_adjoint_ms_without_events": t32_noev * 1e3,
+            "fp32_adjoint_ms_uncaptured": t32_eager * 1e3,
+            "graph": ginfo,
             "fp32_vs_fp64_error": err,
         },
     }

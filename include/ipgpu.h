@@ -65,6 +65,11 @@ int ipg_prof_reset(ipg_ctx* ctx);
 /* Accumulated launches and milliseconds of the kernel named `kernel`
  * (flushes pending events, synchronizing the stream). */
 int ipg_prof_query(ipg_ctx* ctx, const char* kernel, int64_t* launches, double* total_ms);
+/* Enable (default) or disable the replay of captured HIP graphs by the
+ * entry points that support it (ipg_snap_energy_forces*).  With graphs off
+ * every call launches its kernels one by one.  No counterpart in the
+ * reference (a launch-overhead control of this implementation). */
+int ipg_ctx_set_graphs(ipg_ctx* ctx, int on);
 
 /* ------------------------------------------------------------------ */
 /* systems (AtomsBase AbstractSystem, reduced to what the path reads)  */
@@ -194,9 +199,22 @@ int ipg_snap_energy_forces(ipg_snap* snap, const ipg_system* sys, int mode, cons
                            const double* beta, int64_t nbeta, double beta0, int path, double* energy,
                            double* forces);
 /* Device variant: sys arrays, beta (ncoeff*nspecies, expanded), energy[1]
- * and forces[3N] are device pointers. */
+ * and forces[3N] are device pointers.  For a fully periodic system the
+ * step is captured into a HIP graph on the first call and replayed while
+ * the same buffers, box, mode, path and beta0 are passed (an MD loop);
+ * neighbor lists then run at capacities sized from the first call's pair
+ * counts and each call synchronizes once to read a capacity-overflow flag
+ * (an overflowing call is recomputed with exact sizes).  Results equal the
+ * uncaptured call's bit for bit. */
 int ipg_snap_energy_forces_dev(ipg_snap* snap, const ipg_system* sys, int mode, const double* beta, double beta0,
                                int path, double* energy, double* forces);
+/* Captured-step statistics of a snap object: state 0 = no graph, 1 = graph
+ * ready, 2 = capture failed (calls run uncaptured); captures, replays and
+ * overflow recomputations so far.  reason (may be NULL) receives the
+ * capture failure message, truncated to reason_len bytes.  No counterpart
+ * in the reference. */
+int ipg_snap_graph_info(const ipg_snap* snap, int32_t* state, int64_t* captures, int64_t* replays,
+                        int64_t* overflows, char* reason, int64_t reason_len);
 /* Reverse mode of F(x, beta) for force matching (a ChainRulesCore rrule of
  * the force call): given the cotangent gF = dL/dF (3*natoms), returns
  * gbeta = gF^T dF/dbeta (nbeta entries, same layout as beta) and

@@ -36,8 +36,17 @@ CFLAGS = [
 ]
 
 
-def _compile(src, verbose):
+def _stale(src, obj, headers):
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    return any(os.path.getmtime(f) > t for f in [src] + headers)
+
+
+def _compile(src, verbose, incremental=False, headers=()):
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+    if incremental and not _stale(src, obj, list(headers)):
+        return obj, ""
     cmd = [HIPCC] + CFLAGS + ["-c", src, "-o", obj]
     if verbose:
         print(" ".join(cmd), flush=True)
@@ -48,14 +57,17 @@ def _compile(src, verbose):
     return obj, err
 
 
-def build(jobs=None, verbose=False):
+def build(jobs=None, verbose=False, incremental=False):
+    """Compiles every source (incremental=True: only those older than their
+    .o or than any header) and links the library."""
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+    headers = sorted(glob.glob(os.path.join(CSRC, "*.h")) + glob.glob(os.path.join(REPO, "include", "*.h")))
     jobs = jobs or min(8, os.cpu_count() or 4)
     objs = []
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        for obj, err in ex.map(lambda s: _compile(s, verbose), srcs):
+        for obj, err in ex.map(lambda s: _compile(s, verbose, incremental, headers), srcs):
             objs.append(obj)
             if err.strip() and verbose:
                 print(err, file=sys.stderr)
@@ -72,5 +84,6 @@ if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("-j", type=int, default=None)
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--incremental", action="store_true", help="skip sources whose object is up to date")
     a = ap.parse_args()
-    print(build(a.j, a.verbose))
+    print(build(a.j, a.verbose, a.incremental))

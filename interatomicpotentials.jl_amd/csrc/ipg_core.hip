@@ -28,6 +28,7 @@ void* scratch(ipg_ctx* ctx, const char* name, size_t bytes) {
     fail(IPG_ERR_NOMEM, std::string("device allocation of ") + std::to_string(want) + " bytes failed (" + name + ")");
   }
   b.bytes = want;
+  ctx->scratch_gen++;
   return b.ptr;
 }
 
@@ -60,12 +61,26 @@ void flush_prof(ipg_ctx* ctx) {
   ctx->pending.clear();
 }
 
+void add_graph_prof(ipg_ctx* ctx, const std::vector<ipg_ctx::Pending>& ev) {
+  if (ev.empty()) return;
+  IPG_HIP(hipStreamSynchronize(ctx->stream));
+  for (auto& p : ev) {
+    float ms = 0.f;
+    IPG_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+    auto& s = ctx->stats[p.name];
+    s.first += 1;
+    s.second += (double)ms;
+  }
+}
+
 void h2d(ipg_ctx* ctx, void* dst, const void* src, size_t bytes) {
   if (bytes == 0) return;
+  IPG_REQUIRE(!ctx->dry, "internal: host copy in an allocation-only pass");
   IPG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
 }
 void d2h(ipg_ctx* ctx, void* dst, const void* src, size_t bytes) {
   if (bytes == 0) return;
+  IPG_REQUIRE(!ctx->dry, "internal: readback in an allocation-only pass");
   IPG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
 }
 void sync(ipg_ctx* ctx) { IPG_HIP(hipStreamSynchronize(ctx->stream)); }
@@ -366,6 +381,13 @@ int ipg_prof_enable(ipg_ctx* ctx, int on) {
     IPG_REQUIRE(ctx, "ctx is NULL");
     flush_prof(ctx);
     ctx->prof = on != 0;
+  });
+}
+
+int ipg_ctx_set_graphs(ipg_ctx* ctx, int on) {
+  return guard([&] {
+    IPG_REQUIRE(ctx, "ctx is NULL");
+    ctx->graphs = on != 0;
   });
 }
 

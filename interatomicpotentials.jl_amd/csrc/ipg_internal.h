@@ -82,6 +82,10 @@ struct ipg_ctx {
   std::vector<hipEvent_t> free_events;
   std::map<std::string, std::pair<int64_t, double>> stats;
   int num_cus = 256;
+  uint64_t scratch_gen = 0;  // bumped whenever a scratch buffer is (re)allocated
+  bool graphs = true;        // replay captured HIP graphs where a path supports it
+  bool dry = false;          // allocation-only pass: launches are skipped
+  std::vector<Pending>* capture_events = nullptr;  // event pairs of a graph being captured
 };
 
 namespace ipg {
@@ -98,19 +102,28 @@ hipEvent_t get_event(ipg_ctx* ctx);
 
 // Launch wrapper: records HIP events around the launch on the context
 // stream when profiling is on, and checks the launch status.
+// Under stream capture the events become event-record nodes of the graph
+// and are kept by the graph's owner (ctx->capture_events).  In a dry pass
+// nothing is launched (only the scratch allocations of the caller happen).
+inline void record_event(ipg_ctx* ctx, hipEvent_t e) {
+  if (ctx->capture_events) IPG_HIP(hipEventRecordWithFlags(e, ctx->stream, hipEventRecordExternal));
+  else IPG_HIP(hipEventRecord(e, ctx->stream));
+}
+
 template <typename L>
 void launch(ipg_ctx* ctx, const char* name, L&& l) {
+  if (ctx->dry) return;
   hipEvent_t a = nullptr, b = nullptr;
   if (ctx->prof) {
     a = get_event(ctx);
     b = get_event(ctx);
-    IPG_HIP(hipEventRecord(a, ctx->stream));
+    record_event(ctx, a);
   }
   l();
   IPG_HIP(hipGetLastError());
   if (ctx->prof) {
-    IPG_HIP(hipEventRecord(b, ctx->stream));
-    ctx->pending.push_back({name, a, b});
+    record_event(ctx, b);
+    (ctx->capture_events ? *ctx->capture_events : ctx->pending).push_back({name, a, b});
   }
 }
 
@@ -120,6 +133,10 @@ void h2d(ipg_ctx* ctx, void* dst, const void* src, size_t bytes);
 void d2h(ipg_ctx* ctx, void* dst, const void* src, size_t bytes);
 void sync(ipg_ctx* ctx);
 void zero(ipg_ctx* ctx, void* dst, size_t bytes);
+
+// Add the elapsed times of event pairs recorded by a graph replay to the
+// statistics (the events stay owned by the graph).
+void add_graph_prof(ipg_ctx* ctx, const std::vector<ipg_ctx::Pending>& ev);
 
 // exclusive scan of n int32 counts into n+1 int64 offsets (device), returns
 // total (synchronizes when want_total).

@@ -20,6 +20,7 @@ struct DevList {
   int64_t* off = nullptr;  // n+1
   int32_t* nbr = nullptr;  // npairs
   double* disp = nullptr;  // 3*npairs or null
+  bool exact = true;       // npairs is the pair count (false: a capacity, the count is off[n])
 };
 
 DevSystem make_dev_system(const ipg_system* s);  // s holds device pointers
@@ -29,7 +30,7 @@ DevSystem upload_system(ipg_ctx* ctx, const ipg_system* s, const char* tag);  //
 // Storage comes from context scratch buffers named after `tag`.  sorted:
 // j ascending per atom.  want_disp: store get_displacement(x_i, x_j).
 DevList build_list(ipg_ctx* ctx, const DevSystem& s, double rcut, int kind, bool sorted, bool want_disp,
-                   const char* tag);
+                   const char* tag, int64_t cap = 0, int32_t* capflag = nullptr);
 
 // Same, into caller-owned device buffers allocated by the function (for
 // ipg_nlist objects); frees nothing.
@@ -43,6 +44,7 @@ struct SnapListDev {
   int32_t* center = nullptr;   // center atom of each pair
   int64_t* roff = nullptr;     // reverse list offsets (n+1): pairs whose neighbor is the atom
   int32_t* rpair = nullptr;    // reverse list pair indices, ascending
+  int64_t npairs_a = 0;        // size of the stage-A list (count or capacity)
 };
 struct SnapCut {
   int ns;
@@ -50,10 +52,15 @@ struct SnapCut {
   double query;      // inrange radius (sqrt(cutmax) in REFERENCE mode, cutmax otherwise)
   int reference;     // 1: element lookup as utl.jl:70,86 (appear[] of atom and of rank m)
 };
+// capA/capB > 0 select capacity mode (no host readback): the lists hold at
+// most that many pairs, their offsets are clamped to it and capflag gets
+// bit 1 (stage A) or 2 (stage B) when the true count was larger.
 SnapListDev build_snap_list(ipg_ctx* ctx, const DevSystem& s, const int32_t* d_appear, const SnapCut& cut,
-                            bool want_reverse, const char* tag);
+                            bool want_reverse, const char* tag, int64_t capA = 0, int64_t capB = 0,
+                            int32_t* capflag = nullptr);
 
-// center[q] and the reverse list of any CSR list.
+// center[q] and the reverse list of any CSR list (npairs may be a
+// capacity: the count used is off[n] on the device).
 void build_center_and_reverse(ipg_ctx* ctx, int64_t n, int64_t npairs, const int64_t* off, const int32_t* nbr,
                               int32_t* center, int64_t* roff, int32_t* rpair, const char* tag);
 

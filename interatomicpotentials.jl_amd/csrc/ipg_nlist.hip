@@ -140,7 +140,7 @@ __global__ __launch_bounds__(256) void k_list_scan(int64_t n, Grid g, Prefilter 
                                                    const int32_t* __restrict__ cellof,
                                                    const int64_t* __restrict__ cstart, double smax, int kind,
                                                    int32_t* __restrict__ cnt, const int64_t* __restrict__ off,
-                                                   int32_t* __restrict__ nbr) {
+                                                   int32_t* __restrict__ nbr, int64_t cap) {
   const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t k = gt / LP;
   const int l = (int)(gt % LP);
@@ -174,7 +174,7 @@ __global__ __launch_bounds__(256) void k_list_scan(int64_t n, Grid g, Prefilter 
       }
       const uint64_t m = __ballot(hit) & gmask;
       if (FILL) {
-        if (hit) nbr[w + __popcll(m & below)] = j;
+        if (hit && w + __popcll(m & below) < cap) nbr[w + __popcll(m & below)] = j;
         w += __popcll(m);
       } else {
         count += __popcll(m);
@@ -186,10 +186,10 @@ __global__ __launch_bounds__(256) void k_list_scan(int64_t n, Grid g, Prefilter 
 
 // In-segment rank sort: out[off + rank] = in[q], rank = number of smaller
 // entries of the same segment (values within a segment are distinct).
-__global__ void k_seg_rank_sort(int64_t total, int64_t nseg, const int64_t* __restrict__ off,
-                                const int32_t* __restrict__ in, int32_t* __restrict__ out) {
+__global__ void k_seg_rank_sort(int64_t nseg, const int64_t* __restrict__ off, const int32_t* __restrict__ in,
+                                int32_t* __restrict__ out) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= total) return;
+  if (q >= off[nseg]) return;  // off[nseg] is the device total
   int64_t lo = 0, hi = nseg;  // find the segment with off[seg] <= q < off[seg+1]
   while (hi - lo > 1) {
     const int64_t mid = (lo + hi) >> 1;
@@ -331,10 +331,29 @@ DevSystem upload_system(ipg_ctx* ctx, const ipg_system* s, const char* tag) {
   return d;
 }
 
+// Capacity mode: the scanned offsets are clamped to the capacity so every
+// later reader stays inside the buffers, and `bit` is raised in *flag when
+// the true total exceeded it (the results of that call are then discarded).
+__global__ void k_cap_clamp(int64_t n, int64_t* __restrict__ off, int64_t cap, int32_t* __restrict__ flag,
+                            int32_t bit) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  const int64_t v = off[i];
+  if (v > cap) {
+    off[i] = cap;
+    if (i == n) atomicOr(flag, bit);
+  }
+}
+
+static void cap_clamp(ipg_ctx* ctx, int64_t n, int64_t* off, int64_t cap, int32_t* flag, int32_t bit) {
+  IPG_REQUIRE(flag != nullptr, "internal: capacity mode without an overflow flag");
+  launch(ctx, "cap_clamp", [&] { k_cap_clamp<<<grid_for(n + 1, 256), 256, 0, ctx->stream>>>(n, off, cap, flag, bit); });
+}
+
 // Common list builder; alloc(name, bytes) supplies device storage.
 template <typename Alloc>
 static DevList build_list_impl(ipg_ctx* ctx, const DevSystem& s, double rcut, int kind, bool sorted, bool want_disp,
-                               Alloc&& alloc) {
+                               int64_t cap, int32_t* capflag, Alloc&& alloc) {
   IPG_REQUIRE(std::isfinite(rcut) && rcut > 0, "rcutoff must be finite and > 0");
   DevList L;
   L.n = s.n;
@@ -383,10 +402,10 @@ static DevList build_list_impl(ipg_ctx* ctx, const DevSystem& s, double rcut, in
   const int LP = cand >= 192 ? 16 : (cand >= 48 ? 8 : 4);
   const unsigned gl = grid_for(s.n * LP, 256);
   auto scan = [&](auto lp, auto fill, int64_t* offp, int32_t* nbrp) {
-    constexpr int L = decltype(lp)::value;
+    constexpr int LPc = decltype(lp)::value;
     constexpr bool F = decltype(fill)::value;
-    k_list_scan<L, F><<<gl, 256, 0, ctx->stream>>>(s.n, g, pf, catom, spos, sposf, cellof, cstart, smax, kind, cnt,
-                                                   offp, nbrp);
+    k_list_scan<LPc, F><<<gl, 256, 0, ctx->stream>>>(s.n, g, pf, catom, spos, sposf, cellof, cstart, smax, kind, cnt,
+                                                   offp, nbrp, L.npairs);
   };
   auto dispatch = [&](auto fill, int64_t* offp, int32_t* nbrp) {
     if (LP == 16) scan(std::integral_constant<int, 16>(), fill, offp, nbrp);
@@ -394,8 +413,16 @@ static DevList build_list_impl(ipg_ctx* ctx, const DevSystem& s, double rcut, in
     else scan(std::integral_constant<int, 4>(), fill, offp, nbrp);
   };
   launch(ctx, "nlist_count", [&] { dispatch(std::false_type(), nullptr, nullptr); });
-  int64_t np = exclusive_scan(ctx, cnt, s.n, L.off, true);
+  // exact size (reads the total back) or a capacity given by the caller
+  // (no host synchronization; the device total is L.off[n], clamped to the
+  // capacity, and capflag records an overflow)
+  const int64_t np = cap > 0 ? cap : exclusive_scan(ctx, cnt, s.n, L.off, true);
+  if (cap > 0) {
+    exclusive_scan(ctx, cnt, s.n, L.off, false);
+    cap_clamp(ctx, s.n, L.off, cap, capflag, 1);
+  }
   L.npairs = np;
+  L.exact = cap <= 0;
   L.nbr = (int32_t*)alloc("nbr", sizeof(int32_t) * (np > 0 ? np : 1));
   L.disp = want_disp ? (double*)alloc("disp", sizeof(double) * 3 * (np > 0 ? np : 1)) : nullptr;
   int32_t* fill_to = L.nbr;
@@ -403,7 +430,7 @@ static DevList build_list_impl(ipg_ctx* ctx, const DevSystem& s, double rcut, in
   launch(ctx, "nlist_fill", [&] { dispatch(std::true_type(), L.off, fill_to); });
   if (sorted && np > 0)
     launch(ctx, "nlist_sort", [&] {
-      k_seg_rank_sort<<<grid_for(np, 256), 256, 0, ctx->stream>>>(np, s.n, L.off, fill_to, L.nbr);
+      k_seg_rank_sort<<<grid_for(np, 256), 256, 0, ctx->stream>>>(s.n, L.off, fill_to, L.nbr);
     });
   if (want_disp && np > 0)
     launch(ctx, "nlist_disp", [&] { k_list_disp<<<gb, 256, 0, ctx->stream>>>(s.n, L.off, L.nbr, s.pos, s.box, L.disp); });
@@ -411,16 +438,16 @@ static DevList build_list_impl(ipg_ctx* ctx, const DevSystem& s, double rcut, in
 }
 
 DevList build_list(ipg_ctx* ctx, const DevSystem& s, double rcut, int kind, bool sorted, bool want_disp,
-                   const char* tag) {
+                   const char* tag, int64_t cap, int32_t* capflag) {
   std::string t(tag);
-  return build_list_impl(ctx, s, rcut, kind, sorted, want_disp,
+  return build_list_impl(ctx, s, rcut, kind, sorted, want_disp, cap, capflag,
                          [&](const char* what, size_t bytes) { return scratch(ctx, (t + "_" + what).c_str(), bytes); });
 }
 
 DevList build_list_owned(ipg_ctx* ctx, const DevSystem& s, double rcut, int kind, bool sorted, bool want_disp) {
   std::vector<void*> got;
   try {
-    return build_list_impl(ctx, s, rcut, kind, sorted, want_disp, [&](const char*, size_t bytes) {
+    return build_list_impl(ctx, s, rcut, kind, sorted, want_disp, 0, nullptr, [&](const char*, size_t bytes) {
       void* p = nullptr;
       if (hipMalloc(&p, bytes) != hipSuccess) {
         (void)hipGetLastError();
@@ -451,7 +478,7 @@ __global__ void k_snap_stage_b(int64_t n, const int64_t* __restrict__ offA, cons
                                const double* __restrict__ pos, IpgBox box, const int32_t* __restrict__ species,
                                const int32_t* __restrict__ appear, SnapCutDev cut, int32_t* __restrict__ cnt,
                                const int64_t* __restrict__ off, int32_t* __restrict__ nbr, double* __restrict__ R,
-                               double* __restrict__ rvec) {
+                               double* __restrict__ rvec, int64_t cap) {
 #pragma clang fp contract(off)
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -470,6 +497,7 @@ __global__ void k_snap_stage_b(int64_t n, const int64_t* __restrict__ offA, cons
     double rsq = rx * rx + ry * ry + rz * rz;
     if (j != i && rsq <= cut.cutsq[ne * cut.ns + me]) {
       if (FILL) {
+        if (w >= cap) break;  // capacity mode overflow (flagged by the clamp)
         nbr[w] = j;
         R[w] = sqrt(rsq);
         rvec[3 * w] = rx;
@@ -490,16 +518,20 @@ __global__ void k_pair_center(int64_t n, const int64_t* __restrict__ off, int32_
   for (int64_t q = off[i]; q < off[i + 1]; q++) center[q] = (int32_t)i;
 }
 
-__global__ void k_rev_count(int64_t np, const int32_t* __restrict__ nbr, int32_t* __restrict__ rcnt) {
+// The pair count is read on the device (*np = off[n] of the list), so a
+// capacity-sized grid needs no host readback.
+__global__ void k_rev_count(const int64_t* __restrict__ np, const int32_t* __restrict__ nbr,
+                            int32_t* __restrict__ rcnt) {
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= np) return;
+  if (q >= *np) return;
   atomicAdd(&rcnt[nbr[q]], 1);
 }
 
-__global__ void k_rev_place(int64_t np, const int32_t* __restrict__ nbr, const int64_t* __restrict__ roff,
-                            int32_t* __restrict__ rfill, int32_t* __restrict__ rtmp) {
+__global__ void k_rev_place(const int64_t* __restrict__ np, const int32_t* __restrict__ nbr,
+                            const int64_t* __restrict__ roff, int32_t* __restrict__ rfill,
+                            int32_t* __restrict__ rtmp) {
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= np) return;
+  if (q >= *np) return;
   int32_t j = nbr[q];
   int32_t slot = atomicAdd(&rfill[j], 1);
   rtmp[roff[j] + slot] = (int32_t)q;
@@ -514,20 +546,22 @@ void build_center_and_reverse(ipg_ctx* ctx, int64_t n, int64_t npairs, const int
   int32_t* rc = scratch_t<int32_t>(ctx, (t + "_rcnt").c_str(), 2 * n + 2);
   int32_t* rf = rc + n + 1;
   zero(ctx, rc, sizeof(int32_t) * (2 * n + 2));
-  if (npairs > 0) launch(ctx, "rev_count", [&] { k_rev_count<<<gp, 256, 0, ctx->stream>>>(npairs, nbr, rc); });
+  const int64_t* np = off + n;  // device pair count (<= npairs, which may be a capacity)
+  if (npairs > 0) launch(ctx, "rev_count", [&] { k_rev_count<<<gp, 256, 0, ctx->stream>>>(np, nbr, rc); });
   exclusive_scan(ctx, rc, n, roff, false);
   if (npairs > 0) {
     int32_t* rtmp = scratch_t<int32_t>(ctx, (t + "_rtmp").c_str(), npairs);
-    launch(ctx, "rev_place", [&] { k_rev_place<<<gp, 256, 0, ctx->stream>>>(npairs, nbr, roff, rf, rtmp); });
-    launch(ctx, "rev_sort", [&] { k_seg_rank_sort<<<gp, 256, 0, ctx->stream>>>(npairs, n, roff, rtmp, rpair); });
+    launch(ctx, "rev_place", [&] { k_rev_place<<<gp, 256, 0, ctx->stream>>>(np, nbr, roff, rf, rtmp); });
+    launch(ctx, "rev_sort", [&] { k_seg_rank_sort<<<gp, 256, 0, ctx->stream>>>(n, roff, rtmp, rpair); });
   }
 }
 
 SnapListDev build_snap_list(ipg_ctx* ctx, const DevSystem& s, const int32_t* d_appear, const SnapCut& cut,
-                            bool want_reverse, const char* tag) {
+                            bool want_reverse, const char* tag, int64_t capA, int64_t capB, int32_t* capflag) {
   std::string t(tag);
   SnapListDev out;
-  DevList A = build_list(ctx, s, cut.query, KIND_FULL_SELF, true, false, (t + "_A").c_str());
+  DevList A = build_list(ctx, s, cut.query, KIND_FULL_SELF, true, false, (t + "_A").c_str(), capA, capflag);
+  out.npairs_a = A.npairs;
   SnapCutDev cd{};
   cd.ns = cut.ns;
   cd.reference = cut.reference;
@@ -541,9 +575,17 @@ SnapListDev build_snap_list(ipg_ctx* ctx, const DevSystem& s, const int32_t* d_a
   if (n > 0)
     launch(ctx, "snap_list_count", [&] {
       k_snap_stage_b<false><<<gn, 256, 0, ctx->stream>>>(n, A.off, A.nbr, s.pos, s.box, s.species, d_appear, cd,
-                                                         cnt, nullptr, nullptr, nullptr, nullptr);
+                                                         cnt, nullptr, nullptr, nullptr, nullptr, 0);
     });
-  int64_t np = exclusive_scan(ctx, cnt, n, L.off, true);
+  int64_t np;
+  if (capB > 0) {
+    np = capB;
+    exclusive_scan(ctx, cnt, n, L.off, false);
+    cap_clamp(ctx, n, L.off, capB, capflag, 2);
+    L.exact = false;
+  } else {
+    np = exclusive_scan(ctx, cnt, n, L.off, true);
+  }
   L.npairs = np;
   int64_t npa = np > 0 ? np : 1;
   L.nbr = scratch_t<int32_t>(ctx, (t + "_nbr").c_str(), npa);
@@ -552,7 +594,7 @@ SnapListDev build_snap_list(ipg_ctx* ctx, const DevSystem& s, const int32_t* d_a
   if (n > 0 && np > 0)
     launch(ctx, "snap_list_fill", [&] {
       k_snap_stage_b<true><<<gn, 256, 0, ctx->stream>>>(n, A.off, A.nbr, s.pos, s.box, s.species, d_appear, cd,
-                                                        nullptr, L.off, L.nbr, out.R, out.rvec);
+                                                        nullptr, L.off, L.nbr, out.R, out.rvec, np);
     });
   if (want_reverse) {
     out.center = scratch_t<int32_t>(ctx, (t + "_center").c_str(), npa);

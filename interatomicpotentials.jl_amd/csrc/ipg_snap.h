@@ -88,6 +88,8 @@ struct SnapRun {
   const int64_t* roff = nullptr;  // reverse list
   const int32_t* rpair = nullptr;
   const double* tang = nullptr;   // per-atom tangent direction (3n) for Dual runs
+  const int64_t* d_np = nullptr;  // device pair count when npairs is a capacity (else null)
+  double avg_nbr = -1.0;          // mean neighbors per atom for launch heuristics (<0: npairs/n)
 };
 
 // ---------------------------------------------------------------------------
@@ -163,6 +165,36 @@ void snap_launch_dedr(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const S*
 template <typename T>
 void snap_launch_ycoef(ipg_ctx* ctx, const SnapDev& P, const double* beta, T* ycoef);
 void snap_launch_dbdr_plain(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const double* Z, double* dblist);
+// FP32 adjoint Z -> (energy per atom, Y) on the atom-blocked U/Y layout
+// [ceil(n/32)][idxu_max][32][2]; yc: FP64 Y coefficients [nspecies][nblk].
+void snap_launch_zy32(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const float* U, const double* yc,
+                      const double* beta, float* Y, double* eatom);
+
+// A captured energy+force step (lists in capacity mode, no host
+// synchronization inside) replayed while its inputs stay the same buffers.
+struct SnapGraph {
+  hipGraphExec_t exec = nullptr;
+  // what the captured work depends on
+  int64_t n = -1;
+  const double* pos = nullptr;
+  const int32_t* species = nullptr;
+  const int32_t* appear = nullptr;
+  IpgBox box{};
+  int mode = -1, path = -1;
+  const double* beta = nullptr;
+  double beta0 = 0.0;
+  double* energy = nullptr;
+  double* forces = nullptr;
+  bool prof = false;
+  uint64_t gen = 0;  // scratch generation the captured pointers belong to
+  int64_t capA = 0, capB = 0;
+  double avg_nbr = -1.0;
+  int32_t* flag = nullptr;  // capacity overflow bits of the last replay
+  std::vector<ipg_ctx::Pending> events;  // profiling event pairs inside the graph
+  bool disabled[2] = {false, false};     // capture failed (without / with profiling events)
+  std::string reason;
+  int64_t captures = 0, replays = 0, overflows = 0;
+};
 
 }  // namespace ipg
 
@@ -173,4 +205,5 @@ struct ipg_snap {
   void* dmem = nullptr;  // tables
   double* beta_dev = nullptr;  // expanded beta (ncoeff * nspecies), owned
   int64_t beta_cap = 0;
+  ipg::SnapGraph graph;
 };

@@ -387,7 +387,8 @@ static SnapCut make_cut(const SnapHost& h, int mode) {
   return c;
 }
 
-static SnapCall snap_prepare(ipg_snap* sn, const DevSystem& s, int mode, const int32_t* d_appear, bool reverse) {
+static SnapCall snap_prepare(ipg_snap* sn, const DevSystem& s, int mode, const int32_t* d_appear, bool reverse,
+                             int64_t capA = 0, int64_t capB = 0, int32_t* capflag = nullptr) {
   IPG_REQUIRE(mode == IPG_SNAP_LIST_REFERENCE || mode == IPG_SNAP_LIST_CUTOFF, "unknown SNAP list mode");
   SnapCall c;
   SnapCut cut = make_cut(sn->host, mode);
@@ -398,10 +399,11 @@ static SnapCall snap_prepare(ipg_snap* sn, const DevSystem& s, int mode, const i
     zero(sn->ctx, z, sizeof(int32_t) * (s.n > 0 ? s.n : 1));
     ap = z;
   }
-  c.L = build_snap_list(sn->ctx, s, ap, cut, true, "snap_list");
+  c.L = build_snap_list(sn->ctx, s, ap, cut, true, "snap_list", capA, capB, capflag);
   (void)reverse;
   c.R.n = s.n;
   c.R.npairs = c.L.list.npairs;
+  c.R.d_np = c.L.list.exact ? nullptr : c.L.list.off + s.n;
   c.R.off = c.L.list.off;
   c.R.nbr = c.L.list.nbr;
   c.R.center = c.L.center;
@@ -501,12 +503,13 @@ static void energy_forces_dev(ipg_snap* sn, const SnapCall& c, const double* d_b
       snap_launch_zby<double>(ctx, P, c.R, U, d_beta, yc, nullptr, eatom, Y, nullptr, nullptr);
       snap_launch_dedr<double>(ctx, P, c.R, Y, dedr, nullptr);
     } else if (path == IPG_SNAP_ADJOINT_FP32) {
-      float* U = scratch_t<float>(ctx, "snap_U32", 2 * (size_t)P.idxu_max * n);
-      float* Y = scratch_t<float>(ctx, "snap_Y32", 2 * (size_t)P.idxu_max * n);
-      float* yc = scratch_t<float>(ctx, "snap_yc32", (size_t)P.nspecies * P.nblk);
-      snap_launch_ycoef<float>(ctx, P, d_beta, yc);
+      const size_t nblk32 = (size_t)((n + 31) / 32) * 32;  // atom-blocked layout, padded to whole batches
+      float* U = scratch_t<float>(ctx, "snap_U32", 2 * (size_t)P.idxu_max * nblk32);
+      float* Y = scratch_t<float>(ctx, "snap_Y32", 2 * (size_t)P.idxu_max * nblk32);
+      double* yc = scratch_t<double>(ctx, "snap_yc64", (size_t)P.nspecies * P.nblk);
+      snap_launch_ycoef<double>(ctx, P, d_beta, yc);
       snap_launch_ui<float>(ctx, P, c.R, U);
-      snap_launch_zby<float>(ctx, P, c.R, U, d_beta, yc, nullptr, eatom, Y, nullptr, nullptr);
+      snap_launch_zy32(ctx, P, c.R, U, yc, d_beta, Y, eatom);
       snap_launch_dedr<float>(ctx, P, c.R, Y, dedr, nullptr);
     } else {
       fail(IPG_ERR_INVALID, "unknown SNAP path");
@@ -518,6 +521,141 @@ static void energy_forces_dev(ipg_snap* sn, const SnapCall& c, const double* d_b
   }
   if (d_energy)
     launch(ctx, "snap_finish_energy", [&] { k_finish_energy<<<1, 64, 0, ctx->stream>>>(esum, beta0, d_energy); });
+}
+
+// ---------------------------------------------------------------------------
+// captured step: energy + forces of one system, replayed from a HIP graph
+// ---------------------------------------------------------------------------
+static void graph_release(ipg_ctx* ctx, SnapGraph& g) {
+  if (g.exec) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipGraphExecDestroy(g.exec);
+    g.exec = nullptr;
+  }
+  for (auto& p : g.events) {
+    ctx->free_events.push_back(p.a);
+    ctx->free_events.push_back(p.b);
+  }
+  g.events.clear();
+}
+
+static bool graph_matches(const SnapGraph& g, const ipg_ctx* ctx, const DevSystem& s, int mode,
+                          const int32_t* d_appear, const double* d_beta, double beta0, int path, double* d_energy,
+                          double* d_forces) {
+  if (!g.exec || g.n != s.n || g.pos != s.pos || g.species != s.species || g.appear != d_appear) return false;
+  for (int k = 0; k < 3; k++)
+    if (g.box.per[k] != s.box.per[k] || g.box.L[k] != s.box.L[k]) return false;
+  return g.mode == mode && g.path == path && g.beta == d_beta && g.beta0 == beta0 && g.energy == d_energy &&
+         g.forces == d_forces && g.prof == ctx->prof && g.gen == ctx->scratch_gen;
+}
+
+// Energy and forces into device buffers.  For a periodic system with graphs
+// enabled, the first call (and any call whose inputs moved to other buffers
+// or whose pair counts outgrew the capacities) runs with exact list sizes,
+// then sizes capacities from the counts, allocates for them in a dry pass
+// and captures the whole step; later calls replay the graph and read back
+// one overflow flag.  An overflowing replay is recomputed exactly, so every
+// call returns the exact-list result.
+static void snap_energy_forces_call(ipg_snap* sn, const DevSystem& s, int mode, const int32_t* d_appear,
+                                    const double* d_beta, double beta0, int path, double* d_energy,
+                                    double* d_forces) {
+  ipg_ctx* ctx = sn->ctx;
+  SnapGraph& g = sn->graph;
+  const bool periodic = s.box.per[0] && s.box.per[1] && s.box.per[2];
+  const bool use_graph = ctx->graphs && periodic && s.n > 0 && !g.disabled[ctx->prof ? 1 : 0];
+  if (use_graph && graph_matches(g, ctx, s, mode, d_appear, d_beta, beta0, path, d_energy, d_forces)) {
+    IPG_HIP(hipGraphLaunch(g.exec, ctx->stream));
+    int32_t over = 0;
+    d2h(ctx, &over, g.flag, sizeof(int32_t));
+    sync(ctx);
+    g.replays++;
+    if (ctx->prof) add_graph_prof(ctx, g.events);
+    if (over == 0) return;
+    g.overflows++;
+  }
+  // exact run: the result of this call
+  SnapCall c = snap_prepare(sn, s, mode, d_appear, true);
+  energy_forces_dev(sn, c, d_beta, beta0, path, d_energy, d_forces);
+  if (!use_graph) return;
+  graph_release(ctx, g);
+  const int64_t na = c.L.npairs_a, nb = c.L.list.npairs;
+  g.capA = na + na / 8 + 256;
+  g.capB = nb + nb / 8 + 256;
+  g.avg_nbr = (double)nb / (double)s.n;
+  g.flag = scratch_t<int32_t>(ctx, "snap_graph_flag", 1);
+  auto step = [&] {
+    zero(ctx, g.flag, sizeof(int32_t));
+    SnapCall cc = snap_prepare(sn, s, mode, d_appear, true, g.capA, g.capB, g.flag);
+    cc.R.avg_nbr = g.avg_nbr;
+    energy_forces_dev(sn, cc, d_beta, beta0, path, d_energy, d_forces);
+  };
+  // allocation pass at the capacities (nothing is launched)
+  ctx->dry = true;
+  try {
+    step();
+  } catch (...) {
+    ctx->dry = false;
+    throw;
+  }
+  ctx->dry = false;
+  const uint64_t gen = ctx->scratch_gen;
+  if (ctx->prof)  // events for the graph's nodes are created before the capture starts
+    while (ctx->free_events.size() < 256) {
+      hipEvent_t e = nullptr;
+      IPG_HIP(hipEventCreate(&e));
+      ctx->free_events.push_back(e);
+    }
+  std::vector<ipg_ctx::Pending> ev;
+  std::string why;
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    ctx->capture_events = &ev;
+    try {
+      step();
+    } catch (const std::exception& x) {
+      why = x.what();
+    }
+    ctx->capture_events = nullptr;
+    hipError_t e2 = hipStreamEndCapture(ctx->stream, &graph);
+    if (why.empty() && e2 != hipSuccess) why = std::string("hipStreamEndCapture: ") + hipGetErrorString(e2);
+    if (why.empty() && ctx->scratch_gen != gen) why = "scratch reallocated during capture";
+    if (why.empty()) {
+      e2 = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+      if (e2 != hipSuccess) {
+        g.exec = nullptr;
+        why = std::string("hipGraphInstantiate: ") + hipGetErrorString(e2);
+      }
+    }
+    if (graph) (void)hipGraphDestroy(graph);
+  } else {
+    why = std::string("hipStreamBeginCapture: ") + hipGetErrorString(e);
+  }
+  if (!why.empty()) {
+    (void)hipGetLastError();
+    for (auto& p : ev) {
+      ctx->free_events.push_back(p.a);
+      ctx->free_events.push_back(p.b);
+    }
+    g.disabled[ctx->prof ? 1 : 0] = true;
+    g.reason = why;
+    return;  // this call's result came from the exact run
+  }
+  g.events = std::move(ev);
+  g.n = s.n;
+  g.pos = s.pos;
+  g.species = s.species;
+  g.appear = d_appear;
+  g.box = s.box;
+  g.mode = mode;
+  g.path = path;
+  g.beta = d_beta;
+  g.beta0 = beta0;
+  g.energy = d_energy;
+  g.forces = d_forces;
+  g.prof = ctx->prof;
+  g.gen = gen;
+  g.captures++;
 }
 
 }  // namespace ipg
@@ -547,10 +685,31 @@ int ipg_snap_create(ipg_ctx* ctx, const ipg_snap_params* params, ipg_snap** out)
 int ipg_snap_destroy(ipg_snap* sn) {
   return guard([&] {
     if (!sn) return;
-    if (sn->ctx) (void)hipStreamSynchronize(sn->ctx->stream);
+    if (sn->ctx) {
+      (void)hipStreamSynchronize(sn->ctx->stream);
+      graph_release(sn->ctx, sn->graph);
+    }
     if (sn->dmem) (void)hipFree(sn->dmem);
     if (sn->beta_dev) (void)hipFree(sn->beta_dev);
     delete sn;
+  });
+}
+
+int ipg_snap_graph_info(const ipg_snap* sn, int32_t* state, int64_t* captures, int64_t* replays,
+                        int64_t* overflows, char* reason, int64_t reason_len) {
+  return guard([&] {
+    IPG_REQUIRE(sn, "snap is NULL");
+    const SnapGraph& g = sn->graph;
+    const bool off = g.disabled[sn->ctx && sn->ctx->prof ? 1 : 0];
+    if (state) *state = off ? 2 : (g.exec ? 1 : 0);
+    if (captures) *captures = g.captures;
+    if (replays) *replays = g.replays;
+    if (overflows) *overflows = g.overflows;
+    if (reason && reason_len > 0) {
+      const size_t m = std::min<size_t>(g.reason.size(), (size_t)reason_len - 1);
+      std::memcpy(reason, g.reason.data(), m);
+      reason[m] = 0;
+    }
   });
 }
 
@@ -656,9 +815,9 @@ int ipg_snap_energy_forces(ipg_snap* sn, const ipg_system* sys, int mode, const 
     check_species_host(sn, sys, appear);
     const double* db = expand_beta(sn, beta, nbeta);
     DevSystem s = upload_system(ctx, sys, "snap_api");
-    SnapCall c = snap_prepare(sn, s, mode, upload_appear(ctx, appear, s.n), true);
+    const int32_t* d_appear = upload_appear(ctx, appear, s.n);
     double* out = scratch_t<double>(ctx, "snap_api_out", 1 + 3 * std::max<int64_t>(s.n, 1));
-    energy_forces_dev(sn, c, db, beta0, path, out, forces ? out + 1 : nullptr);
+    snap_energy_forces_call(sn, s, mode, d_appear, db, beta0, path, out, forces ? out + 1 : nullptr);
     if (energy) d2h(ctx, energy, out, sizeof(double));
     if (forces) d2h(ctx, forces, out + 1, sizeof(double) * 3 * s.n);
     sync(ctx);
@@ -672,8 +831,7 @@ int ipg_snap_energy_forces_dev(ipg_snap* sn, const ipg_system* sys, int mode, co
     ipg_ctx* ctx = sn->ctx;
     IPG_HIP(hipSetDevice(ctx->device));
     DevSystem s = make_dev_system(sys);
-    SnapCall c = snap_prepare(sn, s, mode, nullptr, true);
-    energy_forces_dev(sn, c, beta, beta0, path, energy, forces);
+    snap_energy_forces_call(sn, s, mode, nullptr, beta, beta0, path, energy, forces);
   });
 }
 

@@ -407,7 +407,10 @@ __global__ __launch_bounds__(64) void k_snap_ui(SnapDev P, SnapRun R, S* __restr
   }
   __syncthreads();
   if (!valid) return;
-  S* out = ulisttot + (int64_t)a * IDXU * 2;
+  // FP32 runs write the atom-blocked layout [a/32][jju][a%32] of the Z/Y kernel
+  constexpr bool BLK = std::is_same<S, float>::value;
+  constexpr int ES = BLK ? 64 : 2;  // element stride
+  S* out = BLK ? ulisttot + ((a >> 5) * (int64_t)IDXU * 32 + (a & 31)) * 2 : ulisttot + (int64_t)a * IDXU * 2;
   for (int e = l; e < IDXU; e += G) {
     int j = 0;
     while (e >= ub_of(j + 1)) j++;
@@ -424,8 +427,8 @@ __global__ __launch_bounds__(64) void k_snap_ui(SnapDev P, SnapRun R, S* __restr
       vi = neg ? acc[g][src][1] : -acc[g][src][1];
     }
     if (ma == mb) vr = vr + (T)P.wself;
-    out[2 * e] = vr;
-    out[2 * e + 1] = vi;
+    out[ES * e] = vr;
+    out[ES * e + 1] = vi;
   }
 }
 
@@ -582,7 +585,7 @@ __global__ __launch_bounds__(256) void k_snap_dedr(SnapDev P, SnapRun R, const S
   using T = typename BaseOf<S>::type;
   constexpr int IDXU = ub_of(J + 1);
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= R.npairs) return;
+  if (q >= (R.d_np ? *R.d_np : R.npairs)) return;
   const int32_t i = R.center[q], j = R.nbr[q];
   const int si = R.species ? R.species[i] : 0, sj = R.species ? R.species[j] : 0;
   typename WideOf<S>::type x, y, z;
@@ -590,7 +593,10 @@ __global__ __launch_bounds__(256) void k_snap_dedr(SnapDev P, SnapRun R, const S
   const double rcut = (P.radii[si] + P.radii[sj]) * P.rcutfac;
   CK<S> c;
   cayley_klein<S, true>(x, y, z, rcut, P.weight[sj], P, c);
-  const S* Yi = Y + (int64_t)i * IDXU * 2;
+  // FP32 runs read the atom-blocked Y of the Z/Y kernel
+  constexpr bool BLK = std::is_same<S, float>::value;
+  constexpr int ES = BLK ? 64 : 2;  // element stride
+  const S* Yi = BLK ? Y + (((int64_t)i >> 5) * IDXU * 32 + (i & 31)) * 2 : Y + (int64_t)i * IDXU * 2;
   // per-row partial sums in S, flushed into A (FP64 for FP32 runs) at the
   // end of every emitted row
   using A = typename AccOf<S>::type;
@@ -599,7 +605,7 @@ __global__ __launch_bounds__(256) void k_snap_dedr(SnapDev P, SnapRun R, const S
   wigner_chain<J, S, 3>(c, [&](auto jc, auto mbc, auto mac, auto hc, auto lc, S u_r, S u_i, auto& dur, auto& dui) {
     constexpr int jl = decltype(jc)::value, mb = decltype(mbc)::value, ma = decltype(mac)::value;
     constexpr int jju = ub_of(jl) + (jl + 1) * mb + ma;
-    S yr = Yi[2 * jju], yi = Yi[2 * jju + 1];
+    S yr = Yi[ES * jju], yi = Yi[ES * jju + 1];
     if constexpr (decltype(hc)::value == 1) {
       yr = (T)0.5 * yr;
       yi = (T)0.5 * yi;
@@ -657,7 +663,7 @@ static void launch_ui_j(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, S* uli
 template <typename S>
 void snap_launch_ui(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, S* ulisttot) {
   if (R.n == 0) return;
-  const double avg = (double)R.npairs / (double)R.n;
+  const double avg = R.avg_nbr >= 0.0 ? R.avg_nbr : (double)R.npairs / (double)R.n;
   const int apw = avg <= 30.0 ? 2 : 1;  // 32 lanes per atom hold up to 32 neighbors in one chunk
   const char* nm = IsDual<S>::value ? "snap_ui_dual" : (sizeof(S) == 4 ? "snap_ui_fp32" : "snap_ui_fp64");
   launch(ctx, nm, [&] {
@@ -715,10 +721,12 @@ void snap_launch_dedr(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const S*
   });
 }
 
-#define IPG_SNAP_INSTANTIATE(S)                                                                                  \
+#define IPG_SNAP_INSTANTIATE_UD(S)                                                                               \
   template void snap_launch_ui<S>(ipg_ctx*, const SnapDev&, const SnapRun&, S*);                                 \
-  template void snap_launch_zby<S>(ipg_ctx*, const SnapDev&, const SnapRun&, const S*, const double*,            \
-                                   const typename BaseOf<S>::type*, double*, double*, S*, S*, double*);          \
   template void snap_launch_dedr<S>(ipg_ctx*, const SnapDev&, const SnapRun&, const S*, double*, double*);
+#define IPG_SNAP_INSTANTIATE(S)                                                                                  \
+  IPG_SNAP_INSTANTIATE_UD(S)                                                                                     \
+  template void snap_launch_zby<S>(ipg_ctx*, const SnapDev&, const SnapRun&, const S*, const double*,            \
+                                   const typename BaseOf<S>::type*, double*, double*, S*, S*, double*);
 
 }  // namespace ipg

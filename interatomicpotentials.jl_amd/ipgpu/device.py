@@ -66,6 +66,11 @@ def prof_enable(on):
     check(_lib.load().ipg_prof_enable(_lib.default_context(), int(on)))
 
 
+def set_graphs(on):
+    """Replay captured HIP graphs in the SNAP energy/force calls (default on)."""
+    check(_lib.load().ipg_ctx_set_graphs(_lib.default_context(), int(on)))
+
+
 def prof_reset():
     check(_lib.load().ipg_prof_reset(_lib.default_context()))
 

@@ -87,6 +87,17 @@ class SNAP:
     def get_species(self):
         return self.species
 
+    def graph_info(self):
+        """Captured-step state of the energy/force calls: dict with state
+        ("none", "ready", "disabled"), captures, replays, overflows, reason."""
+        st = C.c_int32()
+        cap, rep, ovf = C.c_int64(), C.c_int64(), C.c_int64()
+        buf = C.create_string_buffer(512)
+        check(_lib.load().ipg_snap_graph_info(self._h, C.byref(st), C.byref(cap), C.byref(rep), C.byref(ovf),
+                                              buf, 512))
+        return {"state": ("none", "ready", "disabled")[st.value], "captures": cap.value, "replays": rep.value,
+                "overflows": ovf.value, "reason": buf.value.decode()}
+
     def get_rcutoff(self):
         return float(np.max(2.0 * self.radii * self.rcutfac))
 

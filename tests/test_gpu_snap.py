@@ -132,3 +132,24 @@ def test_force_vjp(ipg, orc, precision, tol):
         Xm[a, k] -= h
         fd = (gdotF(Xp) - gdotF(Xm)) / (2 * h)
         assert abs(gp[a, k] - fd) <= max(tol, 1e-6) * max(np.abs(gp).max(), 1.0), (a, k, gp[a, k], fd)
+
+
+@pytest.mark.parametrize("J", [0, 1, 2, 3, 4, 5, 6, 7, 8])
+@pytest.mark.parametrize("bzero", [False, True])
+def test_adjoint_fp32_every_level(ipg, orc, J, bzero):
+    """The FP32 adjoint path (its Z/Y kernel is compiled per twojmax) against
+    the oracle: energy within 1e-5 and forces within 1e-4 of the largest
+    force, relative; 40-atom system so one batch of 32 atoms is partial."""
+    from ipgpu import workloads
+
+    s = workloads.snap_bcc(ncell=3, a=3.16, jitter=0.05, nspecies=2, seed=30 + J)
+    keep = np.arange(40)
+    s = type(s)(s.positions[keep], s.box, s.periodic, [s.symbols[k] for k in keep], unit="angstrom")
+    g, o, sp, (off, nbr, r) = make(ipg, orc, s, J, switch=True, bzero=bzero)
+    rng = np.random.default_rng(100 + J)
+    beta = rng.normal(scale=0.1, size=2 * g.ncoeff)
+    lbp = ipg.snap.LBasisPotential(beta, [0.2], g)
+    ref = o.adjoint_energy_forces(sp, off, nbr, r, beta, 0.2)
+    e32, f32 = ipg.snap.energy_and_force(s, lbp, path="adjoint_fp32")
+    assert abs(e32 - ref["energy"]) <= 1e-5 * max(abs(ref["energy"]), 1.0)
+    assert rel(f32, ref["forces"]) < 1e-4
