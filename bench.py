@@ -85,28 +85,22 @@ def kernel_avg(names):
     return out
 
 
-def cpu_baseline_snap(s, sn_sizes, beta, sp, sample):
-    """Oracle (C restatement of the reference's plain SNAP path: B then
-    per-pair dB then dB . beta), one thread, on the first `sample` atoms."""
+def cpu_baseline_snap(s, beta, sp, sample1):
+    """The oracle (C restatement of the reference's plain SNAP route: per
+    atom B, per pair dB/dr contracted with beta, forces scattered to both
+    atoms), timed on this host: all threads (OpenMP, at most 16) over the
+    whole workload, and one thread over the first `sample1` atoms."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle as O
 
     o = O.Snap(8, 2, 5.0, 0.0, 0.99363, [0.5, 0.5], [1.0, 0.9], switch_flag=True)
     off, nbr, R, r = o.neighborlist(s.positions, s.box, sp, mode=O.SNAP_LIST_CUTOFF)
     n = len(s)
-    M = o.ncoeff * 2
-    B = np.zeros((n, o.ncoeff))
-    dB = np.zeros((n, M, 3))
-    L = O.lib()
-    spc = np.ascontiguousarray(sp, np.int32)
-    t0 = time.perf_counter()
-    L.orc_snap_all_descriptors(C.byref(o.p), o.t, sample, s.positions.ctypes.data, spc.ctypes.data,
-                               off.ctypes.data, nbr.ctypes.data, np.ascontiguousarray(r).ctypes.data,
-                               B.ctypes.data, dB.ctypes.data, None)
-    F = np.einsum("nmk,m->nk", dB, beta)
-    dt = time.perf_counter() - t0
-    del F
-    return sample / dt, dt
+    nth = max(1, min(16, os.cpu_count() or 1))
+    _, _, tn = o.plain_energy_forces(sp, off, nbr, r, beta, nthreads=nth)
+    k = min(sample1, n)
+    _, _, t1 = o.plain_energy_forces(sp, off, nbr, r, beta, nthreads=1, count=k)
+    return {"threads": nth, "rate": n / tn, "seconds": tn, "rate1": k / t1, "seconds1": t1, "sample1": k}
 
 
 def lj_bench(steps, warmup):
@@ -169,6 +163,49 @@ def lj_bench(steps, warmup):
     return res
 
 
+COPY_BYTES = 256 << 20
+
+
+def probe():
+    """Counter-pass workload: the copy kernel's bytes are known (it reads and
+    writes COPY_BYTES), so FETCH_SIZE / WRITE_SIZE of the SNAP kernels can be
+    scaled by what the counters read for it."""
+    import ipgpu
+    from ipgpu._lib import IPG_SNAP_ADJOINT_FP32, check, default_context
+    from ipgpu.device import DeviceArray, set_graphs, synchronize
+
+    src = DeviceArray((COPY_BYTES // 8,), np.float64)
+    dst = DeviceArray((COPY_BYTES // 8,), np.float64)
+    L = ipgpu.load()
+    for _ in range(3):
+        check(L.ipg_device_copy(default_context(), C.c_void_p(dst.ptr), C.c_void_p(src.ptr), COPY_BYTES))
+    synchronize()
+    s, sn, beta, sp, d, dsys = snap_setup()
+    set_graphs(False)
+    for _ in range(3):
+        snap_step(sn, dsys, d, IPG_SNAP_ADJOINT_FP32)
+    synchronize()
+    print(json.dumps({"probe": True, "copy_bytes": COPY_BYTES, "natoms": len(s)}))
+
+
+def read_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (profiles/*_traffic.txt, lines 'kernel: NAME' and
+    'traffic_bytes_per_launch: X'), or None."""
+    import glob
+
+    for fn in sorted(glob.glob(os.path.join(REPO, "profiles", "*_traffic.txt"))):
+        kv = {}
+        with open(fn) as fh:
+            for line in fh:
+                if ":" in line:
+                    k, v = line.split(":", 1)
+                    kv[k.strip()] = v.strip()
+        if kv.get("kernel") == kernel and "traffic_bytes_per_launch" in kv:
+            return float(kv["traffic_bytes_per_launch"])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -176,7 +213,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-lj", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=8192)
+    ap.add_argument("--cpu-sample", type=int, default=2048, help="atoms of the 1-thread CPU baseline")
+    ap.add_argument("--probe", action="store_true",
+                    help="short run for hardware-counter passes: 3 calibration copies of 256 MiB, then 3 uncaptured "
+                         "SNAP FP32 steps")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: every box here has one GPU and this engine has no multi-GPU path "
@@ -189,36 +229,45 @@ def main():
     from ipgpu.device import prof_enable, prof_reset, set_graphs
 
     ipgpu.load()
+    if a.probe:
+        probe()
+        return
     log("setup SNAP workload")
     s, sn, beta, sp, d, dsys = snap_setup()
     n = len(s)
 
-    # FP32 adjoint (value), kernels timed with HIP events over the timed region
+    # FP32 adjoint (value).  Timed region: K steps launched kernel by kernel
+    # on the context stream with HIP events around every launch (the
+    # per-kernel durations of the roofline come from these events; events
+    # inside a captured graph do not time its kernels).
+    step32 = lambda: snap_step(sn, dsys, d, IPG_SNAP_ADJOINT_FP32)
+    set_graphs(False)
+    for _ in range(a.warmup):
+        step32()
     prof_reset()
     prof_enable(True)
-    t32 = time_steps(lambda: snap_step(sn, dsys, d, IPG_SNAP_ADJOINT_FP32), a.steps, a.warmup)
+    t32 = time_steps(step32, a.steps, 0)
     prof_enable(False)
-    k32 = kernel_avg(["snap_ui_fp32", "snap_zy_fp32", "snap_dedr_fp32", "snap_force_gather", "snap_list_count",
-                      "snap_list_fill", "nlist_count", "nlist_fill", "snap_ycoef", "reduce_rows", "scan",
-                      "rev_sort", "cell_sort"])
+    k32 = kernel_avg(["snap_ui_fp32", "snap_zy_fp32", "snap_dedr_fp32", "snap_force_gather", "snap_list_finish",
+                      "snap_pairs", "nlist_count", "nlist_fill", "nlist_sort", "cell_assign", "cell_place",
+                      "cell_rank", "fill_zero", "reduce_rows", "scan"])
     f32 = d["f"].download()
     e32 = float(d["e"].download()[0])
-    ginfo = sn.graph_info()
-    log("fp32 adjoint: %.3f ms/step (graph %s)" % (t32 * 1e3, ginfo))
-    # the same step without profiling events, and without graph replay
-    t32_noev = time_steps(lambda: snap_step(sn, dsys, d, IPG_SNAP_ADJOINT_FP32), a.steps, 1)
-    set_graphs(False)
-    t32_eager = time_steps(lambda: snap_step(sn, dsys, d, IPG_SNAP_ADJOINT_FP32), a.steps, 1)
+    log("fp32 adjoint: %.3f ms/step (uncaptured, HIP events)" % (t32 * 1e3))
+    # the same step replayed from a captured HIP graph (no events)
     set_graphs(True)
-    log("fp32 adjoint without events %.3f ms, uncaptured %.3f ms" % (t32_noev * 1e3, t32_eager * 1e3))
-    # FP64 plain (the reference's route) and FP64 adjoint
+    t32_graph = time_steps(step32, a.steps, max(2, a.warmup))
+    ginfo = sn.graph_info()
+    log("fp32 adjoint graph replay %.3f ms (%s)" % (t32_graph * 1e3, ginfo))
+    # FP64 plain (the reference's route) and FP64 adjoint, uncaptured
+    set_graphs(False)
     steps64 = max(3, a.steps // 4)
     t64 = time_steps(lambda: snap_step(sn, dsys, d, IPG_SNAP_PLAIN_FP64), steps64, 1)
     f64 = d["f"].download()
     e64 = float(d["e"].download()[0])
     t64a = time_steps(lambda: snap_step(sn, dsys, d, IPG_SNAP_ADJOINT_FP64), steps64, 1)
     f64a = d["f"].download()
-
+    set_graphs(True)
     log("fp64 plain: %.3f ms/step, fp64 adjoint %.3f ms/step" % (t64 * 1e3, t64a * 1e3))
     # roofline of the dominant FP32 kernel
     nl = ipgpu.snap.snap_neighborlist(s, sn, list_mode="cutoff")
@@ -254,7 +303,7 @@ def main():
         "config": {"workload": WORKLOAD, "natoms": n, "twojmax": 8, "rcut": 5.0, "species": 2,
                    "pairs": npairs, "switch_flag": True, "list_mode": "cutoff"},
         "roofline": {"bound": "mfma", "achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": ach / PEAK_FP32_TFLOPS, "traffic": None, "kernel": dom,
+                     "frac": ach / PEAK_FP32_TFLOPS, "traffic": read_traffic(dom), "kernel": dom,
                      "kernel_ms": dom_ms, "algorithmic_flops": flops[dom],
                      "note": "VALU-bound FP32 kernel priced against the published dense FP32 peak"},
         "snap": {
@@ -263,8 +312,9 @@ def main():
             "fp64_adjoint_atoms_per_s": n / t64a,
             "speedup_fp32_adjoint_vs_fp64_plain": t64 / t32,
             "kernels_fp32_ms": {k: v[1] for k, v in k32.items()},
-            "fp32_adjoint_ms_without_events": t32_noev * 1e3,
-            "fp32_adjoint_ms_uncaptured": t32_eager * 1e3,
+            "kernel_launches_per_step": {k: v[0] / a.steps for k, v in k32.items()},
+            "fp32_adjoint_ms_graph_replay": t32_graph * 1e3,
+            "fp32_adjoint_atoms_per_s_graph_replay": n / t32_graph,
             "graph": ginfo,
             "fp32_vs_fp64_error": err,
         },
@@ -274,11 +324,13 @@ def main():
         out["lj"] = lj_bench(max(5, a.steps), a.warmup)
     if not a.no_cpu:
         log("CPU baseline (oracle)")
-        sample = min(a.cpu_sample, n)
-        rate, dt = cpu_baseline_snap(s, None, beta, sp, sample)
-        out["cpu_baseline"] = {"value": rate, "unit": "atoms/s", "cores": 1, "kind": "port",
-                               "sample": "oracle plain SNAP 2J=8 E+F (B, per-pair dB, dB.beta) over the first "
-                                         "%d of the 8192 atoms, %.1f s, 1 thread" % (sample, dt)}
+        cb = cpu_baseline_snap(s, beta, sp, a.cpu_sample)
+        out["cpu_baseline"] = {"value": cb["rate"], "unit": "atoms/s", "cores": cb["threads"], "kind": "port",
+                               "sample": "oracle (C restatement of the reference's plain route: B, per-pair dB, "
+                                         "beta . dB) SNAP 2J=8 E+F over all %d atoms on %d OpenMP threads, %.2f s"
+                                         % (n, cb["threads"], cb["seconds"]),
+                               "value_1_thread": cb["rate1"],
+                               "sample_1_thread": "first %d atoms, 1 thread, %.2f s" % (cb["sample1"], cb["seconds1"])}
     print(json.dumps(out))
 
 

@@ -58,6 +58,11 @@ int ipg_device_malloc(ipg_ctx* ctx, size_t bytes, void** dptr);
 int ipg_device_free(ipg_ctx* ctx, void* dptr);
 int ipg_memcpy_h2d(ipg_ctx* ctx, void* dst, const void* src, size_t bytes);
 int ipg_memcpy_d2h(ipg_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* Device-to-device copy by a plain vector-copy kernel ("device_copy"),
+ * enqueued on the context stream; bytes and both pointers multiples of 16.
+ * No counterpart in the reference: a kernel of known HBM traffic for
+ * calibrating hardware byte counters. */
+int ipg_device_copy(ipg_ctx* ctx, void* dst, const void* src, size_t bytes);
 /* Per-kernel timing with HIP events recorded on the context stream around
  * every launch (off by default). */
 int ipg_prof_enable(ipg_ctx* ctx, int on);

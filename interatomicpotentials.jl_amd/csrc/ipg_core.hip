@@ -2,6 +2,8 @@
 // deterministic reductions shared by every entry point.
 #include "ipg_internal.h"
 
+#include <algorithm>
+
 #include <cfloat>
 
 namespace ipg {
@@ -157,7 +159,8 @@ __global__ void k_scan_sums(int64_t* __restrict__ bsum, int64_t nb, int64_t* __r
 }
 
 __global__ void k_scan_final(const int32_t* __restrict__ c, int64_t n, const int64_t* __restrict__ bsum,
-                             const int64_t* __restrict__ total, int64_t* __restrict__ off) {
+                             const int64_t* __restrict__ total, int64_t* __restrict__ off, int64_t cap,
+                             int32_t* __restrict__ flag, int32_t bit) {
   __shared__ int64_t ws[SCAN_BLOCK / 64];
   int64_t i = (int64_t)blockIdx.x * SCAN_BLOCK + threadIdx.x;
   int64_t v = (i < n) ? c[i] : 0;
@@ -178,20 +181,90 @@ __global__ void k_scan_final(const int32_t* __restrict__ c, int64_t n, const int
     }
   }
   __syncthreads();
-  if (i < n) off[i] = bsum[blockIdx.x] + ws[threadIdx.x >> 6] + x - v;
-  if (i == 0) off[n] = *total;
+  if (i < n) {
+    const int64_t e = bsum[blockIdx.x] + ws[threadIdx.x >> 6] + x - v;
+    off[i] = cap > 0 && e > cap ? cap : e;
+  }
+  if (i == 0) {
+    const int64_t t = *total;
+    off[n] = cap > 0 && t > cap ? cap : t;
+    if (cap > 0 && t > cap) atomicOr(flag, bit);
+  }
 }
 
-int64_t exclusive_scan(ipg_ctx* ctx, const int32_t* d_counts, int64_t n, int64_t* d_offsets, bool want_total) {
+// Whole scan in one block (short arrays: one launch instead of three).  Each
+// thread takes SCAN_PER consecutive counts per pass; passes carry the total.
+constexpr int SCAN_PER = 16;
+constexpr int64_t SCAN_SINGLE_MAX = 4 * SCAN_BLOCK * SCAN_PER;
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_single(const int32_t* __restrict__ c, int64_t n,
+                                                            int64_t* __restrict__ off, int64_t* __restrict__ total,
+                                                            int64_t cap, int32_t* __restrict__ flag, int32_t bit) {
+  __shared__ int64_t ws[SCAN_BLOCK / 64];
+  __shared__ int64_t pass_total;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int64_t carry = 0;
+  for (int64_t base = 0; base < n; base += (int64_t)SCAN_BLOCK * SCAN_PER) {
+    const int64_t i0 = base + (int64_t)threadIdx.x * SCAN_PER;
+    int32_t v[SCAN_PER];
+    int64_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; k++) {
+      v[k] = i0 + k < n ? c[i0 + k] : 0;
+      sum += v[k];
+    }
+    int64_t x = sum;  // inclusive wave scan of the thread sums
+    for (int o = 1; o < 64; o <<= 1) {
+      const int64_t y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) ws[wv] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int64_t t = 0;
+      for (int w = 0; w < SCAN_BLOCK / 64; w++) {
+        const int64_t q = ws[w];
+        ws[w] = t;
+        t += q;
+      }
+      pass_total = t;
+    }
+    __syncthreads();
+    int64_t e = carry + ws[wv] + x - sum;
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; k++) {
+      if (i0 + k < n) off[i0 + k] = cap > 0 && e > cap ? cap : e;
+      e += v[k];
+    }
+    carry += pass_total;
+    __syncthreads();  // ws and pass_total are rewritten by the next pass
+  }
+  if (threadIdx.x == 0) {
+    off[n] = cap > 0 && carry > cap ? cap : carry;
+    *total = carry;
+    if (cap > 0 && carry > cap) atomicOr(flag, bit);
+  }
+}
+
+int64_t exclusive_scan(ipg_ctx* ctx, const int32_t* d_counts, int64_t n, int64_t* d_offsets, bool want_total,
+                       int64_t cap, int32_t* flag, int32_t bit) {
+  IPG_REQUIRE(cap <= 0 || flag != nullptr, "internal: capacity scan without an overflow flag");
   int64_t nb = (n + SCAN_BLOCK - 1) / SCAN_BLOCK;
   if (nb < 1) nb = 1;
   int64_t* bsum = scratch_t<int64_t>(ctx, "scan_bsum", nb + 2);
   int64_t* total = bsum + nb;
-  launch(ctx, "scan", [&] {
-    k_scan_block_sums<<<(unsigned)nb, SCAN_BLOCK, 0, ctx->stream>>>(d_counts, n, bsum);
-    k_scan_sums<<<1, SCAN_BLOCK, 0, ctx->stream>>>(bsum, nb, total);
-    k_scan_final<<<(unsigned)nb, SCAN_BLOCK, 0, ctx->stream>>>(d_counts, n, bsum, total, d_offsets);
-  });
+  if (n <= SCAN_SINGLE_MAX) {
+    launch(ctx, "scan", [&] {
+      k_scan_single<<<1, SCAN_BLOCK, 0, ctx->stream>>>(d_counts, n, d_offsets, total, cap, flag, bit);
+    });
+  } else {
+    launch(ctx, "scan", [&] {
+      k_scan_block_sums<<<(unsigned)nb, SCAN_BLOCK, 0, ctx->stream>>>(d_counts, n, bsum);
+      k_scan_sums<<<1, SCAN_BLOCK, 0, ctx->stream>>>(bsum, nb, total);
+      k_scan_final<<<(unsigned)nb, SCAN_BLOCK, 0, ctx->stream>>>(d_counts, n, bsum, total, d_offsets, cap, flag,
+                                                                  bit);
+    });
+  }
   int64_t t = 0;
   if (want_total) {
     d2h(ctx, &t, total, sizeof(int64_t));
@@ -227,7 +300,7 @@ __global__ void k_reduce_rows_p1(const double* __restrict__ rows, int64_t n, int
 }
 
 __global__ void k_reduce_rows_p2(const double* __restrict__ part, int64_t nb, int width, double* __restrict__ out,
-                                 int accumulate) {
+                                 int accumulate, double offset) {
   __shared__ double sh[RED_BLOCK];
   for (int w = 0; w < width; w++) {
     double acc = 0.0;
@@ -238,12 +311,40 @@ __global__ void k_reduce_rows_p2(const double* __restrict__ part, int64_t nb, in
       if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
       __syncthreads();
     }
-    if (threadIdx.x == 0) out[w] = accumulate ? out[w] + sh[0] : sh[0];
+    if (threadIdx.x == 0) out[w] = offset + (accumulate ? out[w] + sh[0] : sh[0]);
     __syncthreads();
   }
 }
 
-void reduce_rows(ipg_ctx* ctx, const double* d_rows, int64_t n, int width, double* d_out, bool accumulate) {
+// Short inputs: the whole reduction in one block (fixed strided partition,
+// fixed tree).
+constexpr int RED1_BLOCK = 1024;
+__global__ __launch_bounds__(RED1_BLOCK) void k_reduce_rows_single(const double* __restrict__ rows, int64_t n,
+                                                                   int width, double* __restrict__ out,
+                                                                   int accumulate, double offset) {
+  __shared__ double sh[RED1_BLOCK];
+  for (int w = 0; w < width; w++) {
+    double acc = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += RED1_BLOCK) acc += rows[i * width + w];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = RED1_BLOCK / 2; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[w] = offset + (accumulate ? out[w] + sh[0] : sh[0]);
+    __syncthreads();
+  }
+}
+
+void reduce_rows(ipg_ctx* ctx, const double* d_rows, int64_t n, int width, double* d_out, bool accumulate,
+                 double offset) {
+  if (n <= 64 * RED1_BLOCK) {
+    launch(ctx, "reduce_rows", [&] {
+      k_reduce_rows_single<<<1, RED1_BLOCK, 0, ctx->stream>>>(d_rows, n, width, d_out, accumulate ? 1 : 0, offset);
+    });
+    return;
+  }
   int64_t nb = (n + 4095) / 4096;
   if (nb < 1) nb = 1;
   if (nb > 1024) nb = 1024;
@@ -252,7 +353,7 @@ void reduce_rows(ipg_ctx* ctx, const double* d_rows, int64_t n, int width, doubl
   double* part = scratch_t<double>(ctx, "reduce_part", nb * (int64_t)width);
   launch(ctx, "reduce_rows", [&] {
     k_reduce_rows_p1<<<(unsigned)nb, RED_BLOCK, 0, ctx->stream>>>(d_rows, n, width, chunk, part);
-    k_reduce_rows_p2<<<1, RED_BLOCK, 0, ctx->stream>>>(part, nb, width, d_out, accumulate ? 1 : 0);
+    k_reduce_rows_p2<<<1, RED_BLOCK, 0, ctx->stream>>>(part, nb, width, d_out, accumulate ? 1 : 0, offset);
   });
 }
 
@@ -381,6 +482,29 @@ int ipg_prof_enable(ipg_ctx* ctx, int on) {
     IPG_REQUIRE(ctx, "ctx is NULL");
     flush_prof(ctx);
     ctx->prof = on != 0;
+  });
+}
+
+// Plain 16-byte-vector copy (grid-stride): a kernel of known traffic, used
+// to calibrate hardware byte counters against.
+__global__ __launch_bounds__(256) void k_device_copy(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                     int64_t n16) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+int ipg_device_copy(ipg_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  return guard([&] {
+    IPG_REQUIRE(ctx && dst && src, "NULL argument");
+    IPG_REQUIRE(bytes % 16 == 0 && ((uintptr_t)dst | (uintptr_t)src) % 16 == 0,
+                "bytes and both pointers must be multiples of 16");
+    IPG_HIP(hipSetDevice(ctx->device));
+    const int64_t n16 = (int64_t)(bytes / 16);
+    if (n16 == 0) return;
+    const unsigned g = (unsigned)std::min<int64_t>((n16 + 255) / 256, (int64_t)ctx->num_cus * 16);
+    launch(ctx, "device_copy", [&] {
+      k_device_copy<<<g, 256, 0, ctx->stream>>>((const uint4*)src, (uint4*)dst, n16);
+    });
   });
 }
 

@@ -140,10 +140,15 @@ void add_graph_prof(ipg_ctx* ctx, const std::vector<ipg_ctx::Pending>& ev);
 
 // exclusive scan of n int32 counts into n+1 int64 offsets (device), returns
 // total (synchronizes when want_total).
-int64_t exclusive_scan(ipg_ctx* ctx, const int32_t* d_counts, int64_t n, int64_t* d_offsets, bool want_total);
+// cap > 0: offsets are clamped to cap and bit is or-ed into *flag when the
+// total exceeds it (capacity mode, no readback).
+int64_t exclusive_scan(ipg_ctx* ctx, const int32_t* d_counts, int64_t n, int64_t* d_offsets, bool want_total,
+                       int64_t cap = 0, int32_t* flag = nullptr, int32_t bit = 0);
 
 // Deterministic sum of n rows of `width` doubles (row-major) into out[width].
-void reduce_rows(ipg_ctx* ctx, const double* d_rows, int64_t n, int width, double* d_out, bool accumulate);
+// out[w] = offset + sum (+ out[w] when accumulate).
+void reduce_rows(ipg_ctx* ctx, const double* d_rows, int64_t n, int width, double* d_out, bool accumulate,
+                 double offset = 0.0);
 
 // Host-side validation of a system description (host pointers).
 void check_system_host(const ipg_system* s);

@@ -21,6 +21,10 @@ struct DevList {
   int32_t* nbr = nullptr;  // npairs
   double* disp = nullptr;  // 3*npairs or null
   bool exact = true;       // npairs is the pair count (false: a capacity, the count is off[n])
+  int ellK = 0;            // > 0: nbr not yet written, rows of width ellK are in ell (counts in cnt)
+  int ellK_req = 0;        // row width the first pass used (0: no row buffer)
+  const int32_t* ell = nullptr;
+  const int32_t* cnt = nullptr;
 };
 
 DevSystem make_dev_system(const ipg_system* s);  // s holds device pointers
@@ -44,20 +48,31 @@ struct SnapListDev {
   int32_t* center = nullptr;   // center atom of each pair
   int64_t* roff = nullptr;     // reverse list offsets (n+1): pairs whose neighbor is the atom
   int32_t* rpair = nullptr;    // reverse list pair indices, ascending
+  int32_t* rev = nullptr;      // index of the reverse pair (j -> i) of each pair (CUTOFF mode)
   int64_t npairs_a = 0;        // size of the stage-A list (count or capacity)
+  int ellK = 0;                // CUTOFF mode: row width of the first pass (0: none)
 };
+// What build_snap_list forms besides nbr/off/rvec/center.
+enum SnapListNeed { SNAPL_R = 1, SNAPL_REVERSE = 2, SNAPL_SYM = 4 };
 struct SnapCut {
   int ns;
   double cutsq[64];  // ns x ns (ns <= 8)
   double query;      // inrange radius (sqrt(cutmax) in REFERENCE mode, cutmax otherwise)
   int reference;     // 1: element lookup as utl.jl:70,86 (appear[] of atom and of rank m)
 };
-// capA/capB > 0 select capacity mode (no host readback): the lists hold at
-// most that many pairs, their offsets are clamped to it and capflag gets
-// bit 1 (stage A) or 2 (stage B) when the true count was larger.
-SnapListDev build_snap_list(ipg_ctx* ctx, const DevSystem& s, const int32_t* d_appear, const SnapCut& cut,
-                            bool want_reverse, const char* tag, int64_t capA = 0, int64_t capB = 0,
-                            int32_t* capflag = nullptr);
+// need: SnapListNeed bits (R: distances; REVERSE: roff/rpair; SYM: rev,
+// CUTOFF mode only -- REFERENCE-mode lists are not symmetric, so there SYM
+// gives roff/rpair).  CUTOFF mode builds the list with one species-aware
+// cell scan; REFERENCE mode runs the two stages of utl.jl.  capA/capB > 0
+// select capacity mode (no host readback): the lists hold at most that many
+// pairs (CUTOFF mode uses capB), their offsets are clamped to it and *flag
+// gets bit 1 (stage A / CUTOFF list) or 2 (stage B) when the true count was
+// larger; bit 4 marks a missing reverse pair.  CUTOFF mode's first pass
+// also writes rows of width ellK (0: estimated from the density) and builds
+// the sorted list from them; bit 8 marks a row wider than ellK.
+SnapListDev build_snap_list(ipg_ctx* ctx, const DevSystem& s, const int32_t* d_appear, const SnapCut& cut, int need,
+                            const char* tag, int64_t capA = 0, int64_t capB = 0, int32_t* flag = nullptr,
+                            int ellK = 0);
 
 // center[q] and the reverse list of any CSR list (npairs may be a
 // capacity: the count used is off[n] on the device).

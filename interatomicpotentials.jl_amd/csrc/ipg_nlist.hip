@@ -58,8 +58,9 @@ __global__ void k_cell_place(int64_t n, const int32_t* __restrict__ cellof, cons
 // in cell order: exact doubles for the membership test and FP32 wrapped
 // coordinates for the prefilter.
 __global__ void k_cell_rank(int64_t n, const int32_t* __restrict__ ctmp, const int32_t* __restrict__ cellof,
-                            const int64_t* __restrict__ cstart, const double* __restrict__ pos, Grid g,
-                            int32_t* __restrict__ catom, double4* __restrict__ spos, float4* __restrict__ sposf) {
+                            const int64_t* __restrict__ cstart, const double* __restrict__ pos,
+                            const int32_t* __restrict__ species, Grid g, int32_t* __restrict__ catom,
+                            double4* __restrict__ spos, float4* __restrict__ sposf) {
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const int32_t i = ctmp[k];
@@ -70,52 +71,101 @@ __global__ void k_cell_rank(int64_t n, const int32_t* __restrict__ ctmp, const i
   const int64_t dst = a + r;
   catom[dst] = i;
   const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
-  spos[dst] = make_double4(x, y, z, 0.0);
+  spos[dst] = make_double4(x, y, z, species ? (double)species[i] : 0.0);  // w: species index
   double w[3] = {x, y, z};
   for (int d = 0; d < 3; d++) w[d] = g.box.per[d] ? w[d] - floor(w[d] / g.box.L[d]) * g.box.L[d] : w[d] - g.lo[d];
   sposf[dst] = make_float4((float)w[0], (float)w[1], (float)w[2], 0.f);
 }
 
+// Candidate atoms of a cell: the 3x3x3 block of cells around it (all cells
+// of a periodic dimension with fewer than 3 cells, clipped at open
+// boundaries), each visited once.  Cells are numbered x fastest and atoms
+// are sorted by cell, so the cells x-1, x, x+1 of one (y, z) row hold one
+// contiguous range of atoms (two ranges where a periodic row wraps): f gets
+// 9 ranges instead of 27 cells.
 template <typename F>
-__device__ __forceinline__ void for_each_cell(const Grid& g, int32_t cell, const int64_t* __restrict__ cstart,
-                                              F&& f) {
-  int cc[3];
-  cc[0] = cell % g.nc[0];
-  cc[1] = (cell / g.nc[0]) % g.nc[1];
-  cc[2] = cell / (g.nc[0] * g.nc[1]);
-  int lo[3], hi[3];
-#pragma unroll
-  for (int d = 0; d < 3; d++) {
-    if (g.full[d]) {
-      lo[d] = 0;
-      hi[d] = g.nc[d] - 1;
-    } else {
-      lo[d] = cc[d] - 1;
-      hi[d] = cc[d] + 1;
-    }
-  }
-  for (int z = lo[2]; z <= hi[2]; z++) {
+__device__ __forceinline__ void for_each_row(const Grid& g, int32_t cell, const int64_t* __restrict__ cstart,
+                                             F&& f) {
+  const int nx = g.nc[0];
+  const int cx = cell % nx, cy = (cell / nx) % g.nc[1], cz = cell / (nx * g.nc[1]);
+  const int ylo = g.full[1] ? 0 : cy - 1, yhi = g.full[1] ? g.nc[1] - 1 : cy + 1;
+  const int zlo = g.full[2] ? 0 : cz - 1, zhi = g.full[2] ? g.nc[2] - 1 : cz + 1;
+  for (int z = zlo; z <= zhi; z++) {
     int zz = z;
     if (!g.full[2]) {
       if (g.box.per[2]) zz = (z + g.nc[2]) % g.nc[2];
       else if (z < 0 || z >= g.nc[2]) continue;
     }
-    for (int y = lo[1]; y <= hi[1]; y++) {
+    for (int y = ylo; y <= yhi; y++) {
       int yy = y;
       if (!g.full[1]) {
         if (g.box.per[1]) yy = (y + g.nc[1]) % g.nc[1];
         else if (y < 0 || y >= g.nc[1]) continue;
       }
-      for (int x = lo[0]; x <= hi[0]; x++) {
-        int xx = x;
-        if (!g.full[0]) {
-          if (g.box.per[0]) xx = (x + g.nc[0]) % g.nc[0];
-          else if (x < 0 || x >= g.nc[0]) continue;
+      const int64_t rb = ((int64_t)zz * g.nc[1] + yy) * nx;
+      if (g.full[0]) {
+        f(cstart[rb], cstart[rb + nx]);
+      } else if (g.box.per[0]) {  // nx >= 3 here
+        if (cx == 0) {
+          f(cstart[rb + nx - 1], cstart[rb + nx]);
+          f(cstart[rb], cstart[rb + 2]);
+        } else if (cx == nx - 1) {
+          f(cstart[rb + nx - 2], cstart[rb + nx]);
+          f(cstart[rb], cstart[rb + 1]);
+        } else {
+          f(cstart[rb + cx - 1], cstart[rb + cx + 2]);
         }
-        const int64_t cid = ((int64_t)zz * g.nc[1] + yy) * g.nc[0] + xx;
-        f(cstart[cid], cstart[cid + 1]);
+      } else {
+        const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < nx ? cx + 1 : nx - 1;
+        f(cstart[rb + x0], cstart[rb + x1 + 1]);
       }
     }
+  }
+}
+
+// Range r (0..17) of the candidates of a cell: row r/2 of the (y, z) rows
+// of for_each_row, run r%2 of that row (the second run exists only where a
+// periodic x row wraps).  Empty ranges come back as s0 == s1.
+__device__ __forceinline__ void cell_range(const Grid& g, int32_t cell, const int64_t* __restrict__ cstart, int r,
+                                           int32_t& s0, int32_t& s1) {
+  s0 = s1 = 0;
+  const int nx = g.nc[0];
+  const int cx = cell % nx, cy = (cell / nx) % g.nc[1], cz = cell / (nx * g.nc[1]);
+  const int nyr = g.full[1] ? g.nc[1] : 3, nzr = g.full[2] ? g.nc[2] : 3;
+  const int row = r >> 1, run = r & 1;
+  if (row >= nyr * nzr) return;
+  int yy = row % nyr, zz = row / nyr;
+  if (!g.full[1]) {
+    yy = cy + yy - 1;
+    if (g.box.per[1]) yy = (yy + g.nc[1]) % g.nc[1];
+    else if (yy < 0 || yy >= g.nc[1]) return;
+  }
+  if (!g.full[2]) {
+    zz = cz + zz - 1;
+    if (g.box.per[2]) zz = (zz + g.nc[2]) % g.nc[2];
+    else if (zz < 0 || zz >= g.nc[2]) return;
+  }
+  const int64_t rb = ((int64_t)zz * g.nc[1] + yy) * nx;
+  int64_t a = 0, b = 0;
+  if (g.full[0]) {
+    if (run == 0) a = rb, b = rb + nx;
+  } else if (g.box.per[0]) {  // nx >= 3 here
+    if (cx == 0) {
+      if (run == 0) a = rb + nx - 1, b = rb + nx;
+      else a = rb, b = rb + 2;
+    } else if (cx == nx - 1) {
+      if (run == 0) a = rb + nx - 2, b = rb + nx;
+      else a = rb, b = rb + 1;
+    } else if (run == 0) {
+      a = rb + cx - 1, b = rb + cx + 2;
+    }
+  } else if (run == 0) {
+    a = rb + (cx > 0 ? cx - 1 : 0);
+    b = rb + (cx + 1 < nx ? cx + 1 : nx - 1) + 1;
+  }
+  if (b > a) {
+    s0 = (int32_t)cstart[a];
+    s1 = (int32_t)cstart[b];
   }
 }
 
@@ -123,6 +173,27 @@ __device__ __forceinline__ bool keep_pair(int kind, int32_t i, int32_t j) {
   if (kind == KIND_HALF) return j > i;
   if (kind == KIND_FULL) return j != i;
   return true;  // KIND_FULL_SELF
+}
+
+// Species-pair cutoffs of the SNAP list (utl.jl:42-56).  reference = 1
+// selects the element lookup of utl.jl:70,86 (two-stage builder only).
+struct SnapCutDev {
+  int ns;
+  int reference;
+  double cutsq[64];
+};
+
+// Second membership test of the SNAP list (utl.jl:84-88): r = x_j - x_i by
+// get_displacement, r.r (left to right, unfused) against the cutoff of the
+// pair's species (w components of the cell-ordered coordinates).
+__device__ __forceinline__ bool snap_cut_ok(const IpgBox& b, const double4& pi, const double4& pj,
+                                            const SnapCutDev& sc) {
+#pragma clang fp contract(off)
+  const double rx = -1.0 * ipg_disp1(pi.x, pj.x, b.L[0], b.per[0]);
+  const double ry = -1.0 * ipg_disp1(pi.y, pj.y, b.L[1], b.per[1]);
+  const double rz = -1.0 * ipg_disp1(pi.z, pj.z, b.L[2], b.per[2]);
+  const double rsq = rx * rx + ry * ry + rz * rz;
+  return rsq <= sc.cutsq[(int)pi.w * sc.ns + (int)pj.w];
 }
 
 struct Prefilter {
@@ -134,13 +205,19 @@ struct Prefilter {
 // taken LP at a time.  A cheap FP32 minimum-image distance rejects far
 // candidates; the rest get the exact FP64 PeriodicEuclidean test.  Hits are
 // placed with a ballot, so the output order is fixed (cell order, then slot).
-template <int LP, bool FILL>
+// MODE: SCAN_COUNT (cnt[i]), SCAN_FILL (rows at off[i], writes stop at cap)
+// or SCAN_ELL (the first K hits of atom i in ell[i*K ...] and their count
+// in cnt[i]; more than K hits raise bit 8 of *flag).
+enum { SCAN_COUNT = 0, SCAN_FILL = 1, SCAN_ELL = 2 };
+template <int LP, int MODE, bool SC>
 __global__ __launch_bounds__(256) void k_list_scan(int64_t n, Grid g, Prefilter pf, const int32_t* __restrict__ catom,
                                                    const double4* __restrict__ spos, const float4* __restrict__ sposf,
                                                    const int32_t* __restrict__ cellof,
                                                    const int64_t* __restrict__ cstart, double smax, int kind,
                                                    int32_t* __restrict__ cnt, const int64_t* __restrict__ off,
-                                                   int32_t* __restrict__ nbr, int64_t cap) {
+                                                   int32_t* __restrict__ nbr, int64_t cap, SnapCutDev sc,
+                                                   int ellK, int32_t* __restrict__ flag) {
+  constexpr bool FILL = MODE == SCAN_FILL;
   const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t k = gt / LP;
   const int l = (int)(gt % LP);
@@ -153,35 +230,60 @@ __global__ __launch_bounds__(256) void k_list_scan(int64_t n, Grid g, Prefilter 
   const float4 qi = sposf[k];
   int64_t w = FILL ? off[i] : 0;
   int count = 0;
-  for_each_cell(g, cellof[i], cstart, [&](int64_t s0, int64_t s1) {
+  // the 18 candidate ranges, computed up front by the group's lanes (all
+  // cstart loads in flight together) and broadcast one at a time
+  constexpr int NR = 18, NS = (NR + LP - 1) / LP;
+  const int32_t cell = cellof[i];
+  int32_t rs[NS], re[NS];
+#pragma unroll
+  for (int t = 0; t < NS; t++) cell_range(g, cell, cstart, l + t * LP, rs[t], re[t]);
+  const int base = lane - l;
+  for (int r = 0; r < NR; r++) {
+    const int t = r / LP;
+    int32_t a0 = rs[0], a1 = re[0];
+#pragma unroll
+    for (int u = 1; u < NS; u++)
+      if (t == u) a0 = rs[u], a1 = re[u];
+    const int32_t s0 = __shfl(a0, base + r % LP, 64), s1 = __shfl(a1, base + r % LP, 64);
     for (int64_t s = s0; s < s1; s += LP) {
       const int64_t sl = s + l;
       bool hit = false;
       int32_t j = -1;
       if (sl < s1) {
         j = catom[sl];
+        const float4 qj = sposf[sl];
+        const double4 pj = spos[sl];
         if (keep_pair(kind, i, j)) {
-          const float4 qj = sposf[sl];
           float dx = qj.x - qi.x, dy = qj.y - qi.y, dz = qj.z - qi.z;
           if (g.box.per[0]) dx -= pf.L[0] * rintf(dx * pf.invL[0]);
           if (g.box.per[1]) dy -= pf.L[1] * rintf(dy * pf.invL[1]);
           if (g.box.per[2]) dz -= pf.L[2] * rintf(dz * pf.invL[2]);
           if (dx * dx + dy * dy + dz * dz <= pf.r2) {
-            const double4 pj = spos[sl];
             hit = ipg_pe_sumsq(g.box, pi.x, pi.y, pi.z, pj.x, pj.y, pj.z) <= smax;
+            if constexpr (SC) hit = hit && snap_cut_ok(g.box, pi, pj, sc);
           }
         }
       }
       const uint64_t m = __ballot(hit) & gmask;
-      if (FILL) {
+      if constexpr (MODE == SCAN_FILL) {
         if (hit && w + __popcll(m & below) < cap) nbr[w + __popcll(m & below)] = j;
         w += __popcll(m);
       } else {
+        if constexpr (MODE == SCAN_ELL) {
+          const int r = count + __popcll(m & below);
+          if (hit && r < ellK) nbr[(int64_t)i * ellK + r] = j;
+        }
         count += __popcll(m);
       }
     }
-  });
-  if (!FILL && l == 0) cnt[i] = count;
+  }
+  if (MODE != SCAN_FILL && l == 0) {
+    if (MODE == SCAN_ELL && count > ellK) {
+      atomicOr(flag, 8);
+      count = ellK;  // the row holds what was written, so every list slot gets an entry
+    }
+    cnt[i] = count;
+  }
 }
 
 // In-segment rank sort: out[off + rank] = in[q], rank = number of smaller
@@ -331,29 +433,15 @@ DevSystem upload_system(ipg_ctx* ctx, const ipg_system* s, const char* tag) {
   return d;
 }
 
-// Capacity mode: the scanned offsets are clamped to the capacity so every
-// later reader stays inside the buffers, and `bit` is raised in *flag when
-// the true total exceeded it (the results of that call are then discarded).
-__global__ void k_cap_clamp(int64_t n, int64_t* __restrict__ off, int64_t cap, int32_t* __restrict__ flag,
-                            int32_t bit) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i > n) return;
-  const int64_t v = off[i];
-  if (v > cap) {
-    off[i] = cap;
-    if (i == n) atomicOr(flag, bit);
-  }
-}
-
-static void cap_clamp(ipg_ctx* ctx, int64_t n, int64_t* off, int64_t cap, int32_t* flag, int32_t bit) {
-  IPG_REQUIRE(flag != nullptr, "internal: capacity mode without an overflow flag");
-  launch(ctx, "cap_clamp", [&] { k_cap_clamp<<<grid_for(n + 1, 256), 256, 0, ctx->stream>>>(n, off, cap, flag, bit); });
-}
-
 // Common list builder; alloc(name, bytes) supplies device storage.
+// ellK > 0 (with sc, sorted): the first pass also writes each atom's hits
+// into rows of width ellK (scratch "nl_ell"); when no row overflowed the
+// returned list has L.ellK = ellK and nbr is left for the caller to fill
+// from the rows (the fill and sort passes are skipped).  An overflow sets
+// bit 8 of *capflag; exact sizing then runs the fill and sort passes.
 template <typename Alloc>
 static DevList build_list_impl(ipg_ctx* ctx, const DevSystem& s, double rcut, int kind, bool sorted, bool want_disp,
-                               int64_t cap, int32_t* capflag, Alloc&& alloc) {
+                               int64_t cap, int32_t* capflag, const SnapCutDev* sc, Alloc&& alloc, int ellK = 0) {
   IPG_REQUIRE(std::isfinite(rcut) && rcut > 0, "rcutoff must be finite and > 0");
   DevList L;
   L.n = s.n;
@@ -393,41 +481,86 @@ static DevList build_list_impl(ipg_ctx* ctx, const DevSystem& s, double rcut, in
   exclusive_scan(ctx, ccount, g.ncell, cstart, false);
   launch(ctx, "cell_place", [&] { k_cell_place<<<gb, 256, 0, ctx->stream>>>(s.n, cellof, cstart, cfill, ctmp); });
   launch(ctx, "cell_rank", [&] {
-    k_cell_rank<<<gb, 256, 0, ctx->stream>>>(s.n, ctmp, cellof, cstart, s.pos, g, catom, spos, sposf);
+    k_cell_rank<<<gb, 256, 0, ctx->stream>>>(s.n, ctmp, cellof, cstart, s.pos, s.species, g, catom, spos, sposf);
   });
-  // lanes per atom from the expected candidate count
-  int ncellv = 1;
-  for (int d = 0; d < 3; d++) ncellv *= g.full[d] ? g.nc[d] : std::min(3, g.nc[d]);
-  const double cand = (double)s.n / (double)g.ncell * ncellv;
-  const int LP = cand >= 192 ? 16 : (cand >= 48 ? 8 : 4);
+  // lanes per atom from the expected candidates per contiguous row range
+  const double cand = (double)s.n / (double)g.ncell * (g.full[0] ? g.nc[0] : std::min(3, g.nc[0]));
+  const int LP = cand >= 40 ? 64 : (cand >= 18 ? 32 : (cand >= 9 ? 16 : 8));
   const unsigned gl = grid_for(s.n * LP, 256);
-  auto scan = [&](auto lp, auto fill, int64_t* offp, int32_t* nbrp) {
+  SnapCutDev scd{};
+  if (sc) scd = *sc;
+  if (ellK < 0) {
+    // row width from the expected neighbor count at the grid's mean density
+    double vol = (double)g.ncell;
+    for (int d = 0; d < 3; d++) vol *= g.w[d];
+    const double expect = (double)s.n / vol * 4.18879020478639 * rcut * rcut * rcut;
+    ellK = (int)std::min<double>(4096.0, std::ceil((1.5 * expect + 32.0) / 32.0) * 32.0);
+  }
+  const bool ell_mode = ellK > 0 && sc && sorted;
+  L.ellK_req = ell_mode ? ellK : 0;
+  int32_t* ell = ell_mode ? scratch_t<int32_t>(ctx, "nl_ell", (size_t)s.n * ellK) : nullptr;
+  auto scan = [&](auto lp, auto mode, int64_t* offp, int32_t* nbrp) {
     constexpr int LPc = decltype(lp)::value;
-    constexpr bool F = decltype(fill)::value;
-    k_list_scan<LPc, F><<<gl, 256, 0, ctx->stream>>>(s.n, g, pf, catom, spos, sposf, cellof, cstart, smax, kind, cnt,
-                                                   offp, nbrp, L.npairs);
+    constexpr int M = decltype(mode)::value;
+    if (sc)
+      k_list_scan<LPc, M, true><<<gl, 256, 0, ctx->stream>>>(s.n, g, pf, catom, spos, sposf, cellof, cstart, smax,
+                                                             kind, cnt, offp, nbrp, L.npairs, scd, ellK, capflag);
+    else
+      k_list_scan<LPc, M, false><<<gl, 256, 0, ctx->stream>>>(s.n, g, pf, catom, spos, sposf, cellof, cstart, smax,
+                                                              kind, cnt, offp, nbrp, L.npairs, scd, ellK, capflag);
   };
-  auto dispatch = [&](auto fill, int64_t* offp, int32_t* nbrp) {
-    if (LP == 16) scan(std::integral_constant<int, 16>(), fill, offp, nbrp);
-    else if (LP == 8) scan(std::integral_constant<int, 8>(), fill, offp, nbrp);
-    else scan(std::integral_constant<int, 4>(), fill, offp, nbrp);
+  auto dispatch = [&](auto mode, int64_t* offp, int32_t* nbrp) {
+    if (LP == 64) scan(std::integral_constant<int, 64>(), mode, offp, nbrp);
+    else if (LP == 32) scan(std::integral_constant<int, 32>(), mode, offp, nbrp);
+    else if (LP == 16) scan(std::integral_constant<int, 16>(), mode, offp, nbrp);
+    else scan(std::integral_constant<int, 8>(), mode, offp, nbrp);
   };
-  launch(ctx, "nlist_count", [&] { dispatch(std::false_type(), nullptr, nullptr); });
+  if (ell_mode) {
+    IPG_REQUIRE(capflag != nullptr, "internal: row-buffer scan without a flag");
+    launch(ctx, "nlist_count", [&] { dispatch(std::integral_constant<int, SCAN_ELL>(), nullptr, ell); });
+  } else {
+    launch(ctx, "nlist_count", [&] { dispatch(std::integral_constant<int, SCAN_COUNT>(), nullptr, nullptr); });
+  }
   // exact size (reads the total back) or a capacity given by the caller
   // (no host synchronization; the device total is L.off[n], clamped to the
   // capacity, and capflag records an overflow)
-  const int64_t np = cap > 0 ? cap : exclusive_scan(ctx, cnt, s.n, L.off, true);
-  if (cap > 0) {
-    exclusive_scan(ctx, cnt, s.n, L.off, false);
-    cap_clamp(ctx, s.n, L.off, cap, capflag, 1);
+  int64_t np = cap > 0 ? cap : exclusive_scan(ctx, cnt, s.n, L.off, true);
+  if (cap > 0) exclusive_scan(ctx, cnt, s.n, L.off, false, cap, capflag, 1);
+  L.exact = cap <= 0;
+  if (ell_mode) {
+    // The rows are used unless one overflowed.  With a capacity the
+    // overflow bit makes the caller recompute; with exact sizing it is read
+    // here, and the list is recounted and filled without the row buffer.
+    bool ok = true;
+    if (cap <= 0) {
+      int32_t f = 0;
+      d2h(ctx, &f, capflag, sizeof(int32_t));
+      sync(ctx);
+      ok = (f & 8) == 0;
+    }
+    if (ok) {
+      L.npairs = np;
+      L.nbr = (int32_t*)alloc("nbr", sizeof(int32_t) * (np > 0 ? np : 1));
+      L.disp = want_disp ? (double*)alloc("disp", sizeof(double) * 3 * (np > 0 ? np : 1)) : nullptr;
+      L.ellK = ellK;
+      L.ell = ell;
+      L.cnt = cnt;
+      return L;
+    }
+    launch(ctx, "nlist_count", [&] { dispatch(std::integral_constant<int, SCAN_COUNT>(), nullptr, nullptr); });
+    np = exclusive_scan(ctx, cnt, s.n, L.off, true);
+    std::vector<int32_t> hc((size_t)s.n);
+    d2h(ctx, hc.data(), cnt, sizeof(int32_t) * s.n);
+    sync(ctx);
+    const int widest = *std::max_element(hc.begin(), hc.end());
+    L.ellK_req = (int)std::min<int64_t>(4096, ((int64_t)widest + 16 + 31) / 32 * 32);  // width for a next call
   }
   L.npairs = np;
-  L.exact = cap <= 0;
   L.nbr = (int32_t*)alloc("nbr", sizeof(int32_t) * (np > 0 ? np : 1));
   L.disp = want_disp ? (double*)alloc("disp", sizeof(double) * 3 * (np > 0 ? np : 1)) : nullptr;
   int32_t* fill_to = L.nbr;
   if (sorted && np > 0) fill_to = scratch_t<int32_t>(ctx, "nl_unsorted", np);
-  launch(ctx, "nlist_fill", [&] { dispatch(std::true_type(), L.off, fill_to); });
+  launch(ctx, "nlist_fill", [&] { dispatch(std::integral_constant<int, SCAN_FILL>(), L.off, fill_to); });
   if (sorted && np > 0)
     launch(ctx, "nlist_sort", [&] {
       k_seg_rank_sort<<<grid_for(np, 256), 256, 0, ctx->stream>>>(s.n, L.off, fill_to, L.nbr);
@@ -440,14 +573,14 @@ static DevList build_list_impl(ipg_ctx* ctx, const DevSystem& s, double rcut, in
 DevList build_list(ipg_ctx* ctx, const DevSystem& s, double rcut, int kind, bool sorted, bool want_disp,
                    const char* tag, int64_t cap, int32_t* capflag) {
   std::string t(tag);
-  return build_list_impl(ctx, s, rcut, kind, sorted, want_disp, cap, capflag,
+  return build_list_impl(ctx, s, rcut, kind, sorted, want_disp, cap, capflag, nullptr,
                          [&](const char* what, size_t bytes) { return scratch(ctx, (t + "_" + what).c_str(), bytes); });
 }
 
 DevList build_list_owned(ipg_ctx* ctx, const DevSystem& s, double rcut, int kind, bool sorted, bool want_disp) {
   std::vector<void*> got;
   try {
-    return build_list_impl(ctx, s, rcut, kind, sorted, want_disp, 0, nullptr, [&](const char*, size_t bytes) {
+    return build_list_impl(ctx, s, rcut, kind, sorted, want_disp, 0, nullptr, nullptr, [&](const char*, size_t bytes) {
       void* p = nullptr;
       if (hipMalloc(&p, bytes) != hipSuccess) {
         (void)hipGetLastError();
@@ -467,12 +600,6 @@ DevList build_list_owned(ipg_ctx* ctx, const DevSystem& s, double rcut, int kind
 // (radius `query`, self included, so the enumerate index m is the rank in
 // that list).
 // ---------------------------------------------------------------------------
-struct SnapCutDev {
-  int ns;
-  int reference;
-  double cutsq[64];
-};
-
 template <bool FILL>
 __global__ void k_snap_stage_b(int64_t n, const int64_t* __restrict__ offA, const int32_t* __restrict__ nbrA,
                                const double* __restrict__ pos, IpgBox box, const int32_t* __restrict__ species,
@@ -556,17 +683,177 @@ void build_center_and_reverse(ipg_ctx* ctx, int64_t n, int64_t npairs, const int
   }
 }
 
-SnapListDev build_snap_list(ipg_ctx* ctx, const DevSystem& s, const int32_t* d_appear, const SnapCut& cut,
-                            bool want_reverse, const char* tag, int64_t capA, int64_t capB, int32_t* capflag) {
+// Per pair of a sorted CUTOFF-mode SNAP list: center atom, r = x_j - x_i
+// (get_displacement, as utl.jl:84), optionally |r| and the index of the
+// reverse pair (j -> i) found by binary search in j's sorted row.  A pair
+// whose reverse is missing (possible only when r.r lies within an ulp of the
+// cutoff under periodic wrapping) raises bit 4 of *flag.
+__global__ void k_snapc_pairs(int64_t n, const int64_t* __restrict__ off, const int32_t* __restrict__ nbr,
+                              const double* __restrict__ pos, IpgBox box, int32_t* __restrict__ center,
+                              double* __restrict__ rvec, double* __restrict__ R, int32_t* __restrict__ rev,
+                              int32_t* __restrict__ flag) {
+#pragma clang fp contract(off)
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= off[n]) return;  // device pair count (the grid may cover a capacity)
+  int64_t lo = 0, hi = n;   // center: off[lo] <= q < off[lo + 1]
+  while (hi - lo > 1) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (off[mid] <= q) lo = mid;
+    else hi = mid;
+  }
+  const int32_t i = (int32_t)lo, j = nbr[q];
+  center[q] = i;
+  const double rx = -1.0 * ipg_disp1(pos[3 * i], pos[3 * j], box.L[0], box.per[0]);
+  const double ry = -1.0 * ipg_disp1(pos[3 * i + 1], pos[3 * j + 1], box.L[1], box.per[1]);
+  const double rz = -1.0 * ipg_disp1(pos[3 * i + 2], pos[3 * j + 2], box.L[2], box.per[2]);
+  rvec[3 * q] = rx;
+  rvec[3 * q + 1] = ry;
+  rvec[3 * q + 2] = rz;
+  if (R) R[q] = sqrt(rx * rx + ry * ry + rz * rz);
+  if (rev) {
+    int64_t a = off[j], b = off[j + 1];
+    const int64_t end = b;
+    while (a < b) {  // first entry >= i in j's ascending row
+      const int64_t m = (a + b) >> 1;
+      if (nbr[m] < i) a = m + 1;
+      else b = m;
+    }
+    if (a < end && nbr[a] == i) {
+      rev[q] = (int32_t)a;
+    } else {
+      rev[q] = (int32_t)q;
+      atomicOr(flag, 4);
+    }
+  }
+}
+
+// Sorted CUTOFF-mode rows from the first pass's row buffer: each group of LP
+// lanes stages atom i's hits in LDS, ranks them (the sorted order of the
+// inrange list), and writes nbr, center, r = x_j - x_i (get_displacement,
+// utl.jl:84), optionally |r|, and the index of the reverse pair (the rank
+// of i in j's row).  A missing reverse pair raises bit 4 of *flag.
+template <int LP>
+__global__ __launch_bounds__(256) void k_snapc_finish(int64_t n, int K, const int32_t* __restrict__ ell,
+                                                      const int32_t* __restrict__ cnt,
+                                                      const int64_t* __restrict__ off, const double* __restrict__ pos,
+                                                      IpgBox box, int32_t* __restrict__ nbr,
+                                                      int32_t* __restrict__ center, double* __restrict__ rvec,
+                                                      double* __restrict__ R, int32_t* __restrict__ rev,
+                                                      int32_t* __restrict__ flag) {
+#pragma clang fp contract(off)
+  extern __shared__ int32_t rowbuf[];  // [256 / LP][K]
+  const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = gt / LP;
+  const int l = (int)(gt % LP);
+  int32_t* row = rowbuf + (threadIdx.x / LP) * K;
+  const bool act = i < n;
+  int c = 0;
+  int64_t a = 0, b = 0;
+  if (act) {
+    c = min(cnt[i], K);
+    a = off[i];
+    b = off[i + 1];
+    for (int k = l; k < c; k += LP) row[k] = ell[i * K + k];
+  }
+  __syncthreads();
+  if (!act) return;
+  const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+  for (int k = l; k < c; k += LP) {
+    const int32_t j = row[k];
+    int r = 0;
+    for (int m = 0; m < c; m++) r += row[m] < j;
+    const int64_t q = a + r;
+    if (q >= b) continue;  // row cut by a capacity (flagged by the scan)
+    nbr[q] = j;
+    center[q] = (int32_t)i;
+    const double rx = -1.0 * ipg_disp1(xi, pos[3 * j], box.L[0], box.per[0]);
+    const double ry = -1.0 * ipg_disp1(yi, pos[3 * j + 1], box.L[1], box.per[1]);
+    const double rz = -1.0 * ipg_disp1(zi, pos[3 * j + 2], box.L[2], box.per[2]);
+    rvec[3 * q] = rx;
+    rvec[3 * q + 1] = ry;
+    rvec[3 * q + 2] = rz;
+    if (R) R[q] = sqrt(rx * rx + ry * ry + rz * rz);
+    if (rev) {
+      const int cj = min(cnt[j], K);
+      const int32_t* rj = ell + (int64_t)j * K;
+      int rr = 0;
+      bool found = false;
+      for (int m = 0; m < cj; m++) {
+        const int32_t v = rj[m];
+        rr += v < (int32_t)i;
+        found = found || v == (int32_t)i;
+      }
+      const int64_t qr = off[j] + rr;
+      if (found && qr < off[j + 1]) {
+        rev[q] = (int32_t)qr;
+      } else {
+        rev[q] = (int32_t)q;
+        atomicOr(flag, 4);
+      }
+    }
+  }
+}
+
+SnapListDev build_snap_list(ipg_ctx* ctx, const DevSystem& s, const int32_t* d_appear, const SnapCut& cut, int need,
+                            const char* tag, int64_t capA, int64_t capB, int32_t* flag, int ellK) {
   std::string t(tag);
   SnapListDev out;
+  const int64_t n = s.n;
+  if (!cut.reference) {
+    // CUTOFF mode: both membership tests of utl.jl:69-90 in one species-aware
+    // cell scan, rows sorted by j (the order of the stage-A inrange list)
+    SnapCutDev cd{};
+    cd.ns = cut.ns;
+    for (int k = 0; k < cut.ns * cut.ns; k++) cd.cutsq[k] = cut.cutsq[k];
+    IPG_REQUIRE(flag != nullptr, "internal: SNAP list without a flag word");
+    DevList L = build_list_impl(
+        ctx, s, cut.query, KIND_FULL, true, false, capB, flag, &cd,
+        [&](const char* what, size_t bytes) { return scratch(ctx, (t + "_" + what).c_str(), bytes); },
+        ellK != 0 ? ellK : -1);
+    out.list = L;
+    out.npairs_a = L.npairs;
+    out.ellK = L.ellK_req;
+    IPG_REQUIRE(L.npairs < ((int64_t)1 << 31), "too many SNAP pairs for 32-bit pair indices");
+    const int64_t npa = L.npairs > 0 ? L.npairs : 1;
+    out.center = scratch_t<int32_t>(ctx, (t + "_center").c_str(), npa);
+    out.rvec = scratch_t<double>(ctx, (t + "_rvec").c_str(), 3 * npa);
+    out.R = (need & SNAPL_R) ? scratch_t<double>(ctx, (t + "_R").c_str(), npa) : nullptr;
+    out.rev = (need & SNAPL_SYM) ? scratch_t<int32_t>(ctx, (t + "_rev").c_str(), npa) : nullptr;
+    IPG_REQUIRE(!out.rev || flag, "internal: reverse-pair search without a flag");
+    if (n > 0 && L.npairs > 0 && L.ellK > 0) {
+      const int K = L.ellK;
+      const int LP = K >= 64 ? 64 : 32;
+      const size_t sm = sizeof(int32_t) * (size_t)(256 / LP) * K;
+      IPG_REQUIRE(sm <= 64 * 1024, "internal: neighbor row buffer too wide");
+      launch(ctx, "snap_list_finish", [&] {
+        if (LP == 64)
+          k_snapc_finish<64><<<grid_for(n * 64, 256), 256, sm, ctx->stream>>>(
+              n, K, L.ell, L.cnt, L.off, s.pos, s.box, L.nbr, out.center, out.rvec, out.R, out.rev, flag);
+        else
+          k_snapc_finish<32><<<grid_for(n * 32, 256), 256, sm, ctx->stream>>>(
+              n, K, L.ell, L.cnt, L.off, s.pos, s.box, L.nbr, out.center, out.rvec, out.R, out.rev, flag);
+      });
+    } else if (n > 0 && L.npairs > 0) {
+      launch(ctx, "snap_pairs", [&] {
+        k_snapc_pairs<<<grid_for(L.npairs, 256), 256, 0, ctx->stream>>>(n, L.off, L.nbr, s.pos, s.box, out.center,
+                                                                       out.rvec, out.R, out.rev, flag);
+      });
+    }
+    if (need & SNAPL_REVERSE) {
+      out.roff = scratch_t<int64_t>(ctx, (t + "_roff").c_str(), n + 1);
+      out.rpair = scratch_t<int32_t>(ctx, (t + "_rpair").c_str(), npa);
+      build_center_and_reverse(ctx, n, L.npairs, L.off, L.nbr, nullptr, out.roff, out.rpair, t.c_str());
+    }
+    return out;
+  }
+  const bool want_reverse = (need & (SNAPL_REVERSE | SNAPL_SYM)) != 0;
+  int32_t* capflag = flag;
   DevList A = build_list(ctx, s, cut.query, KIND_FULL_SELF, true, false, (t + "_A").c_str(), capA, capflag);
   out.npairs_a = A.npairs;
   SnapCutDev cd{};
   cd.ns = cut.ns;
   cd.reference = cut.reference;
   for (int k = 0; k < cut.ns * cut.ns; k++) cd.cutsq[k] = cut.cutsq[k];
-  int64_t n = s.n;
   DevList& L = out.list;
   L.n = n;
   L.off = scratch_t<int64_t>(ctx, (t + "_off").c_str(), n + 1);
@@ -580,8 +867,7 @@ SnapListDev build_snap_list(ipg_ctx* ctx, const DevSystem& s, const int32_t* d_a
   int64_t np;
   if (capB > 0) {
     np = capB;
-    exclusive_scan(ctx, cnt, n, L.off, false);
-    cap_clamp(ctx, n, L.off, capB, capflag, 2);
+    exclusive_scan(ctx, cnt, n, L.off, false, capB, capflag, 2);
     L.exact = false;
   } else {
     np = exclusive_scan(ctx, cnt, n, L.off, true);

@@ -88,6 +88,7 @@ struct SnapRun {
   const int64_t* roff = nullptr;  // reverse list
   const int32_t* rpair = nullptr;
   const double* tang = nullptr;   // per-atom tangent direction (3n) for Dual runs
+  const int32_t* rev = nullptr;   // reverse pair of each pair (symmetric CUTOFF list) or null
   const int64_t* d_np = nullptr;  // device pair count when npairs is a capacity (else null)
   double avg_nbr = -1.0;          // mean neighbors per atom for launch heuristics (<0: npairs/n)
 };
@@ -166,9 +167,9 @@ template <typename T>
 void snap_launch_ycoef(ipg_ctx* ctx, const SnapDev& P, const double* beta, T* ycoef);
 void snap_launch_dbdr_plain(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const double* Z, double* dblist);
 // FP32 adjoint Z -> (energy per atom, Y) on the atom-blocked U/Y layout
-// [ceil(n/32)][idxu_max][32][2]; yc: FP64 Y coefficients [nspecies][nblk].
-void snap_launch_zy32(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const float* U, const double* yc,
-                      const double* beta, float* Y, double* eatom);
+// [ceil(n/32)][idxu_max][32][2]; beta expanded [nspecies][ncoeff].
+void snap_launch_zy32(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const float* U, const double* beta, float* Y,
+                      double* eatom);
 
 // A captured energy+force step (lists in capacity mode, no host
 // synchronization inside) replayed while its inputs stay the same buffers.
@@ -188,8 +189,9 @@ struct SnapGraph {
   bool prof = false;
   uint64_t gen = 0;  // scratch generation the captured pointers belong to
   int64_t capA = 0, capB = 0;
+  int ellK = 0;  // neighbor row width of the CUTOFF list's first pass
   double avg_nbr = -1.0;
-  int32_t* flag = nullptr;  // capacity overflow bits of the last replay
+  int32_t* flag = nullptr;  // bits of the last replay: 1, 2, 8 capacity; 4 missing reverse pair
   std::vector<ipg_ctx::Pending> events;  // profiling event pairs inside the graph
   bool disabled[2] = {false, false};     // capture failed (without / with profiling events)
   std::string reason;

@@ -5,6 +5,7 @@
 // Clebsch-Gordan coefficients from the Racah closed form); the derived
 // tables (Y coefficients, dB half-sum terms) are this engine's own.
 #include "ipg_snap.h"
+#include "ipg_snap_kernels.h"
 
 #include <algorithm>
 
@@ -273,26 +274,64 @@ static void snap_upload(ipg_snap* sn) {
 // ---------------------------------------------------------------------------
 // gathers (deterministic: fixed pair order per atom, no atomics)
 // ---------------------------------------------------------------------------
-__global__ void k_force_gather(int64_t n, const int64_t* __restrict__ off, const int64_t* __restrict__ roff,
-                               const int32_t* __restrict__ rpair, const double* __restrict__ dedr,
-                               double* __restrict__ F) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// F_i = sum_(i->j) dE_i/dr_ij - sum_(j->i) dE_j/dr_ji (snap.jl:264-265),
+// 16 lanes per atom.  Lane l takes the l-th, (l+16)-th, ... entry of the
+// atom's own row, then of its reverse row, and the lanes are summed in a
+// fixed tree.  SYM: the reverse row is rev[] of the own row; otherwise
+// roff/rpair.  For a symmetric list both give the same entries in the same
+// order, so the two variants agree bit for bit.
+constexpr int FG_LANES = 16;
+template <bool SYM>
+__global__ __launch_bounds__(256) void k_force_gather16(int64_t n, const int64_t* __restrict__ off,
+                                                        const int32_t* __restrict__ rev,
+                                                        const int64_t* __restrict__ roff,
+                                                        const int32_t* __restrict__ rpair,
+                                                        const double* __restrict__ dedr, double* __restrict__ F) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = t / FG_LANES;
+  const int l = (int)(t % FG_LANES);
+  if (i >= n) return;  // whole 16-lane groups leave together
   double fx = 0, fy = 0, fz = 0;
-  for (int64_t q = off[i]; q < off[i + 1]; q++) {  // snap.jl:264 (forces[i] += dE_i/dr_ij)
+  const int64_t a = off[i], b = off[i + 1];
+  for (int64_t q = a + l; q < b; q += FG_LANES) {
     fx += dedr[3 * q];
     fy += dedr[3 * q + 1];
     fz += dedr[3 * q + 2];
   }
-  for (int64_t r = roff[i]; r < roff[i + 1]; r++) {  // snap.jl:265 (forces[j] -= dE_i/dr_ij)
-    const int64_t q = rpair[r];
-    fx -= dedr[3 * q];
-    fy -= dedr[3 * q + 1];
-    fz -= dedr[3 * q + 2];
+  if constexpr (SYM) {
+    for (int64_t q = a + l; q < b; q += FG_LANES) {
+      const int64_t r = rev[q];
+      fx -= dedr[3 * r];
+      fy -= dedr[3 * r + 1];
+      fz -= dedr[3 * r + 2];
+    }
+  } else {
+    for (int64_t k = roff[i] + l; k < roff[i + 1]; k += FG_LANES) {
+      const int64_t r = rpair[k];
+      fx -= dedr[3 * r];
+      fy -= dedr[3 * r + 1];
+      fz -= dedr[3 * r + 2];
+    }
   }
-  F[3 * i] = fx;
-  F[3 * i + 1] = fy;
-  F[3 * i + 2] = fz;
+  fx = group_sum<FG_LANES>(fx);
+  fy = group_sum<FG_LANES>(fy);
+  fz = group_sum<FG_LANES>(fz);
+  if (l == 0) {
+    F[3 * i] = fx;
+    F[3 * i + 1] = fy;
+    F[3 * i + 2] = fz;
+  }
+}
+
+static void force_gather(ipg_ctx* ctx, const SnapRun& R, const double* dedr, double* F) {
+  if (R.n == 0) return;
+  const unsigned g = grid_for(R.n * FG_LANES, 256);
+  launch(ctx, "snap_force_gather", [&] {
+    if (R.rev)
+      k_force_gather16<true><<<g, 256, 0, ctx->stream>>>(R.n, R.off, R.rev, nullptr, nullptr, dedr, F);
+    else
+      k_force_gather16<false><<<g, 256, 0, ctx->stream>>>(R.n, R.off, nullptr, R.roff, R.rpair, dedr, F);
+  });
 }
 
 // dB[a] (M x 3, blocks by the species of the pair's center atom).
@@ -366,6 +405,7 @@ __global__ void k_finish_energy(const double* __restrict__ sum, double beta0, do
 struct SnapCall {
   SnapListDev L;
   SnapRun R;
+  int32_t* flag = nullptr;  // device bits: 1, 2, 8 capacity overflow; 4 missing reverse pair
 };
 
 static SnapCut make_cut(const SnapHost& h, int mode) {
@@ -387,10 +427,13 @@ static SnapCut make_cut(const SnapHost& h, int mode) {
   return c;
 }
 
-static SnapCall snap_prepare(ipg_snap* sn, const DevSystem& s, int mode, const int32_t* d_appear, bool reverse,
-                             int64_t capA = 0, int64_t capB = 0, int32_t* capflag = nullptr) {
+// need: SnapListNeed bits.  The call's flag word is cleared first.
+static SnapCall snap_prepare(ipg_snap* sn, const DevSystem& s, int mode, const int32_t* d_appear, int need,
+                             int64_t capA = 0, int64_t capB = 0, int ellK = 0) {
   IPG_REQUIRE(mode == IPG_SNAP_LIST_REFERENCE || mode == IPG_SNAP_LIST_CUTOFF, "unknown SNAP list mode");
   SnapCall c;
+  c.flag = scratch_t<int32_t>(sn->ctx, "snap_flag", 1);
+  zero(sn->ctx, c.flag, sizeof(int32_t));
   SnapCut cut = make_cut(sn->host, mode);
   const int32_t* ap = d_appear ? d_appear : s.species;
   if (cut.reference && !ap) {
@@ -399,8 +442,7 @@ static SnapCall snap_prepare(ipg_snap* sn, const DevSystem& s, int mode, const i
     zero(sn->ctx, z, sizeof(int32_t) * (s.n > 0 ? s.n : 1));
     ap = z;
   }
-  c.L = build_snap_list(sn->ctx, s, ap, cut, true, "snap_list", capA, capB, capflag);
-  (void)reverse;
+  c.L = build_snap_list(sn->ctx, s, ap, cut, need, "snap_list", capA, capB, c.flag, ellK);
   c.R.n = s.n;
   c.R.npairs = c.L.list.npairs;
   c.R.d_np = c.L.list.exact ? nullptr : c.L.list.off + s.n;
@@ -411,8 +453,14 @@ static SnapCall snap_prepare(ipg_snap* sn, const DevSystem& s, int mode, const i
   c.R.species = s.species;
   c.R.roff = c.L.roff;
   c.R.rpair = c.L.rpair;
+  c.R.rev = c.L.rev;
   return c;
 }
+
+// List bits an energy/force path needs: the adjoint gathers use the
+// reverse-pair index of the symmetric CUTOFF list, the plain path the
+// general reverse rows (its dB gather goes by the center's species).
+static int ef_need(int path) { return path == IPG_SNAP_PLAIN_FP64 ? SNAPL_REVERSE : SNAPL_SYM; }
 
 static void check_species_host(const ipg_snap* sn, const ipg_system* sys, const int32_t* appear) {
   for (int64_t i = 0; i < sys->natoms; i++) {
@@ -491,7 +539,8 @@ static void energy_forces_dev(ipg_snap* sn, const SnapCall& c, const double* d_b
     gather_db(sn, c, dbl, dB);
     launch(ctx, "snap_force_from_db",
            [&] { k_force_from_db<<<grid_for(3 * n, 256), 256, 0, ctx->stream>>>(n, M, dB, d_beta, F); });
-    reduce_rows(ctx, eatom, n, 1, esum, false);
+    if (d_energy) reduce_rows(ctx, eatom, n, 1, d_energy, false, beta0);  // E = beta0 + sum_i E_i
+    return;
   } else {
     double* dedr = scratch_t<double>(ctx, "snap_dedr", 3 * std::max<int64_t>(c.R.npairs, 1));
     if (path == IPG_SNAP_ADJOINT_FP64) {
@@ -506,21 +555,35 @@ static void energy_forces_dev(ipg_snap* sn, const SnapCall& c, const double* d_b
       const size_t nblk32 = (size_t)((n + 31) / 32) * 32;  // atom-blocked layout, padded to whole batches
       float* U = scratch_t<float>(ctx, "snap_U32", 2 * (size_t)P.idxu_max * nblk32);
       float* Y = scratch_t<float>(ctx, "snap_Y32", 2 * (size_t)P.idxu_max * nblk32);
-      double* yc = scratch_t<double>(ctx, "snap_yc64", (size_t)P.nspecies * P.nblk);
-      snap_launch_ycoef<double>(ctx, P, d_beta, yc);
       snap_launch_ui<float>(ctx, P, c.R, U);
-      snap_launch_zy32(ctx, P, c.R, U, yc, d_beta, Y, eatom);
+      snap_launch_zy32(ctx, P, c.R, U, d_beta, Y, eatom);
       snap_launch_dedr<float>(ctx, P, c.R, Y, dedr, nullptr);
     } else {
       fail(IPG_ERR_INVALID, "unknown SNAP path");
     }
-    launch(ctx, "snap_force_gather", [&] {
-      k_force_gather<<<grid_for(n, 256), 256, 0, ctx->stream>>>(n, c.R.off, c.R.roff, c.R.rpair, dedr, F);
-    });
-    reduce_rows(ctx, eatom, n, 1, esum, false);
+    force_gather(ctx, c.R, dedr, F);
+    if (d_energy) reduce_rows(ctx, eatom, n, 1, d_energy, false, beta0);  // E = beta0 + sum_i E_i
+    return;
   }
   if (d_energy)
     launch(ctx, "snap_finish_energy", [&] { k_finish_energy<<<1, 64, 0, ctx->stream>>>(esum, beta0, d_energy); });
+}
+
+// A CUTOFF-mode list with a pair whose reverse is missing: rebuild the
+// general reverse rows and redo the adjoint force gather from the stored
+// dE_i/dr_ij of the same call.
+static void fix_asymmetric(ipg_snap* sn, SnapCall& c, double* d_forces) {
+  ipg_ctx* ctx = sn->ctx;
+  const int64_t n = c.R.n, np = c.L.list.npairs, npa = np > 0 ? np : 1;
+  c.L.roff = scratch_t<int64_t>(ctx, "snap_list_roff", n + 1);
+  c.L.rpair = scratch_t<int32_t>(ctx, "snap_list_rpair", npa);
+  build_center_and_reverse(ctx, n, np, c.L.list.off, c.L.list.nbr, nullptr, c.L.roff, c.L.rpair, "snap_list");
+  c.R.roff = c.L.roff;
+  c.R.rpair = c.L.rpair;
+  c.R.rev = nullptr;
+  double* dedr = scratch_t<double>(ctx, "snap_dedr", 3 * std::max<int64_t>(np, 1));
+  double* F = d_forces ? d_forces : scratch_t<double>(ctx, "snap_Fscratch", 3 * std::max<int64_t>(n, 1));
+  force_gather(ctx, c.R, dedr, F);
 }
 
 // ---------------------------------------------------------------------------
@@ -563,6 +626,7 @@ static void snap_energy_forces_call(ipg_snap* sn, const DevSystem& s, int mode, 
   SnapGraph& g = sn->graph;
   const bool periodic = s.box.per[0] && s.box.per[1] && s.box.per[2];
   const bool use_graph = ctx->graphs && periodic && s.n > 0 && !g.disabled[ctx->prof ? 1 : 0];
+  bool keep_graph = false;
   if (use_graph && graph_matches(g, ctx, s, mode, d_appear, d_beta, beta0, path, d_energy, d_forces)) {
     IPG_HIP(hipGraphLaunch(g.exec, ctx->stream));
     int32_t over = 0;
@@ -571,21 +635,28 @@ static void snap_energy_forces_call(ipg_snap* sn, const DevSystem& s, int mode, 
     g.replays++;
     if (ctx->prof) add_graph_prof(ctx, g.events);
     if (over == 0) return;
-    g.overflows++;
+    if (over & 11) g.overflows++;   // a list or a neighbor row outgrew its capacity
+    keep_graph = (over & 11) == 0;  // only reverse pairs were missing: the capacities hold
   }
   // exact run: the result of this call
-  SnapCall c = snap_prepare(sn, s, mode, d_appear, true);
+  SnapCall c = snap_prepare(sn, s, mode, d_appear, ef_need(path));
   energy_forces_dev(sn, c, d_beta, beta0, path, d_energy, d_forces);
-  if (!use_graph) return;
+  if (c.R.rev) {
+    int32_t f = 0;
+    d2h(ctx, &f, c.flag, sizeof(int32_t));
+    sync(ctx);
+    if (f & 4) fix_asymmetric(sn, c, d_forces);
+  }
+  if (!use_graph || keep_graph) return;
   graph_release(ctx, g);
   const int64_t na = c.L.npairs_a, nb = c.L.list.npairs;
   g.capA = na + na / 8 + 256;
   g.capB = nb + nb / 8 + 256;
   g.avg_nbr = (double)nb / (double)s.n;
-  g.flag = scratch_t<int32_t>(ctx, "snap_graph_flag", 1);
+  g.ellK = c.L.ellK;
+  g.flag = c.flag;  // snap_prepare clears it at the start of every step
   auto step = [&] {
-    zero(ctx, g.flag, sizeof(int32_t));
-    SnapCall cc = snap_prepare(sn, s, mode, d_appear, true, g.capA, g.capB, g.flag);
+    SnapCall cc = snap_prepare(sn, s, mode, d_appear, ef_need(path), g.capA, g.capB, g.ellK);
     cc.R.avg_nbr = g.avg_nbr;
     energy_forces_dev(sn, cc, d_beta, beta0, path, d_energy, d_forces);
   };
@@ -731,7 +802,7 @@ int ipg_snap_neighborlist(ipg_snap* sn, const ipg_system* sys, int mode, const i
     IPG_HIP(hipSetDevice(ctx->device));
     check_species_host(sn, sys, appear);
     DevSystem s = upload_system(ctx, sys, "snap_api");
-    SnapCall c = snap_prepare(sn, s, mode, upload_appear(ctx, appear, s.n), false);
+    SnapCall c = snap_prepare(sn, s, mode, upload_appear(ctx, appear, s.n), SNAPL_R);
     d2h(ctx, offsets, c.L.list.off, sizeof(int64_t) * (s.n + 1));
     *npairs = c.L.list.npairs;
     if (c.L.list.npairs > cap) {
@@ -753,7 +824,7 @@ int ipg_snap_local_descriptors(ipg_snap* sn, const ipg_system* sys, int mode, co
     IPG_HIP(hipSetDevice(ctx->device));
     check_species_host(sn, sys, appear);
     DevSystem s = upload_system(ctx, sys, "snap_api");
-    SnapCall c = snap_prepare(sn, s, mode, upload_appear(ctx, appear, s.n), false);
+    SnapCall c = snap_prepare(sn, s, mode, upload_appear(ctx, appear, s.n), 0);
     if (s.n == 0) return;
     double* dB = scratch_t<double>(ctx, "snap_Bout", (size_t)sn->host.ncoeff * s.n);
     run_descriptors_fp64(sn, c, nullptr, dB, nullptr, nullptr, false);
@@ -770,7 +841,7 @@ int ipg_snap_all_descriptors(ipg_snap* sn, const ipg_system* sys, int mode, cons
     IPG_HIP(hipSetDevice(ctx->device));
     check_species_host(sn, sys, appear);
     DevSystem s = upload_system(ctx, sys, "snap_api");
-    SnapCall c = snap_prepare(sn, s, mode, upload_appear(ctx, appear, s.n), true);
+    SnapCall c = snap_prepare(sn, s, mode, upload_appear(ctx, appear, s.n), SNAPL_REVERSE);
     const int nc = sn->host.ncoeff, M = nc * sn->host.nspecies;
     if (s.n == 0) {
       if (W) std::fill(W, W + 6 * M, 0.0);
@@ -846,7 +917,7 @@ int ipg_snap_force_vjp(ipg_snap* sn, const ipg_system* sys, int mode, const int3
     for (int64_t i = 0; i < 3 * sys->natoms; i++) IPG_REQUIRE(std::isfinite(gF[i]), "gF must be finite");
     const double* db = expand_beta(sn, beta, nbeta);
     DevSystem s = upload_system(ctx, sys, "snap_api");
-    SnapCall c = snap_prepare(sn, s, mode, upload_appear(ctx, appear, s.n), true);
+    SnapCall c = snap_prepare(sn, s, mode, upload_appear(ctx, appear, s.n), SNAPL_REVERSE);
     const SnapDev& P = sn->dev;
     const int64_t n = s.n;
     const int nc = P.ncoeff, M = nc * P.nspecies;
@@ -880,9 +951,7 @@ int ipg_snap_force_vjp(ipg_snap* sn, const ipg_system* sys, int mode, const int3
       snap_launch_dedr<S>(ctx, P, c.R, Y, dedr, dedr_t);
     }
     double* gp = scratch_t<double>(ctx, "snap_vjp_gpos", 3 * n);
-    launch(ctx, "snap_force_gather", [&] {
-      k_force_gather<<<grid_for(n, 256), 256, 0, ctx->stream>>>(n, c.R.off, c.R.roff, c.R.rpair, dedr_t, gp);
-    });
+    force_gather(ctx, c.R, dedr_t, gp);
     double* gb = scratch_t<double>(ctx, "snap_vjp_gbeta", M);
     reduce_rows(ctx, tanB, n, M, gb, false);
     std::vector<double> hgb(M);

@@ -350,29 +350,110 @@ __device__ __forceinline__ void load_pair(const SnapRun& R, int64_t q, int32_t i
   }
 }
 
+// Position of entry (j, mb, ma) in wigner_chain's emission order, and the
+// number of emitted entries.
+constexpr int emit_index(int J, int j, int mb, int ma) {
+  int k = 0;
+  for (int MB = 0; 2 * MB <= J; MB++) {
+    for (int a = 0; a <= MB; a++) {  // level 2MB, columns 0..MB
+      if (j == 2 * MB && mb == MB && ma == a) return k;
+      k++;
+    }
+    for (int JJ = 2 * MB + 1; JJ <= J; JJ++)
+      for (int a = 0; a <= JJ; a++) {
+        if (j == JJ && mb == MB && ma == a) return k;
+        k++;
+      }
+  }
+  return k;  // past the end: the count
+}
+template <int J>
+struct EmitJju {  // jju of the k-th emitted entry
+  int v[emit_index(J, -1, -1, -1)] = {};
+  constexpr EmitJju() {
+    int k = 0;
+    for (int MB = 0; 2 * MB <= J; MB++) {
+      for (int a = 0; a <= MB; a++) v[k++] = ub_of(2 * MB) + (2 * MB + 1) * MB + a;
+      for (int JJ = 2 * MB + 1; JJ <= J; JJ++)
+        for (int a = 0; a <= JJ; a++) v[k++] = ub_of(JJ) + (JJ + 1) * MB + a;
+    }
+  }
+};
+
+// Transposed sum over the G lanes of a group of the NV values every lane
+// holds (NV a power of two, NV <= G): exchanges at xor distances G/2, G/4,
+// ... each keep half of the values (the upper half where the lane has that
+// bit), then plain butterfly adds cover the remaining distances.  Lane l
+// ends with the group total of value l >> log2(G/NV) in v[0]; lanes that
+// differ only in the low log2(G/NV) bits hold the same total.
+template <int M, int D, typename T>
+__device__ __forceinline__ void transpose_sum(T* v, int l) {
+  if constexpr (D >= 1) {
+    if constexpr (M > 1) {
+      const bool hi = (l & D) != 0;
+      static_for<0, M / 2>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        const T send = hi ? v[k] : v[k + M / 2];
+        const T keep = hi ? v[k + M / 2] : v[k];
+        v[k] = keep + __shfl_xor(send, D, 64);
+      });
+      transpose_sum<M / 2, D / 2>(v, l);
+    } else {
+      v[0] = v[0] + __shfl_xor(v[0], D, 64);
+      transpose_sum<1, D / 2>(v, l);
+    }
+  }
+}
+
+// Components of a U value for the lane sums: (re, im), and for Dual runs
+// the tangents after them.
+template <typename S> struct UComp { static constexpr int NC = 2; };
+template <typename T> struct UComp<Dual<T>> { static constexpr int NC = 4; };
+template <typename T>
+__device__ __forceinline__ void put_comp(T* buf, int B, int e, T r, T i) {
+  buf[e] = r;
+  buf[B + e] = i;
+}
+template <typename T>
+__device__ __forceinline__ void put_comp(T* buf, int B, int e, Dual<T> r, Dual<T> i) {
+  buf[e] = r.v;
+  buf[B + e] = i.v;
+  buf[2 * B + e] = r.t;
+  buf[3 * B + e] = i.t;
+}
+
 // ---------------------------------------------------------------------------
 // K1: ulisttot (compute_ui.jl:5-44, zero_uarraytot :131-152, add_uarraytot
 // :155-175).  One wave holds APW atoms; the G = 64/APW lanes of an atom take
-// one neighbor each (looping over chunks of G neighbors) and every emitted
-// entry is summed over the group with a fixed butterfly, so the result is
-// deterministic.  The right half is filled from the left half by the
-// inversion symmetry, and w_self sits on the diagonal.
+// one neighbor each (looping over chunks of G neighbors) and run the Wigner
+// row chain in registers.  The emitted entries w_j u_j are buffered in
+// batches of B entries; each full batch is summed over the group with one
+// transposed reduction, after which every lane owns one (entry, component)
+// of the batch and keeps its running total in a register across chunks.
+// The totals are written to LDS once per atom; the right half is filled
+// from the left half by the inversion symmetry and wself is added on the
+// diagonal.  All sums run in a fixed order, so the result is deterministic.
 // ---------------------------------------------------------------------------
 template <int J, typename S, int APW>
 __global__ __launch_bounds__(64) void k_snap_ui(SnapDev P, SnapRun R, S* __restrict__ ulisttot) {
   using T = typename BaseOf<S>::type;
   constexpr int G = 64 / APW;
   constexpr int IDXU = ub_of(J + 1);
+  constexpr int NC = UComp<S>::NC;
+  constexpr int B = 32 * 4 / (NC * (int)sizeof(T));  // entries per batch: NC * B values of 32 VGPRs
+  constexpr int NV = NC * B;
+  constexpr int NE = emit_index(J, -1, -1, -1);
+  constexpr int NB = (NE + B - 1) / B;
+  constexpr int SHIFT = (G == 64 ? 6 : 5) - (NV == 64 ? 6 : NV == 32 ? 5 : NV == 16 ? 4 : 3);
+  static_assert(NV <= G, "batch larger than the lane group");
   __shared__ S acc[APW][IDXU][2];
   const int lane = threadIdx.x;
   const int g = lane / G, l = lane % G;
   const int64_t a = (int64_t)blockIdx.x * APW + g;
   const bool valid = a < R.n;
-  for (int e = l; e < IDXU; e += G) {
-    acc[g][e][0] = S((T)0);
-    acc[g][e][1] = S((T)0);
-  }
-  __syncthreads();
+  T tot[NB];
+#pragma unroll
+  for (int b = 0; b < NB; b++) tot[b] = (T)0;
   const int64_t q0 = valid ? R.off[a] : 0, q1 = valid ? R.off[a + 1] : 0;
   int nch = (int)((q1 - q0 + G - 1) / G);
   nch = max(max(__builtin_amdgcn_readlane(nch, 0), __builtin_amdgcn_readlane(nch, 16)),
@@ -395,15 +476,35 @@ __global__ __launch_bounds__(64) void k_snap_ui(SnapDev P, SnapRun R, S* __restr
       c.sfac = S((T)0);
     }
     const S w = c.sfac;
+    T buf[NV];
     wigner_chain<J, S, 0>(c, [&](auto jc, auto mbc, auto mac, auto, auto, S u_r, S u_i, auto&, auto&) {
-      constexpr int jju = ub_of(decltype(jc)::value) + (decltype(jc)::value + 1) * decltype(mbc)::value +
-                          decltype(mac)::value;
-      const S vr = group_sum<G>(w * u_r), vi = group_sum<G>(w * u_i);
-      if (l == jju % G) {
-        acc[g][jju][0] += vr;
-        acc[g][jju][1] += vi;
+      constexpr int k = emit_index(J, decltype(jc)::value, decltype(mbc)::value, decltype(mac)::value);
+      constexpr int e = k % B, b = k / B;
+      put_comp(buf, B, e, w * u_r, w * u_i);
+      if constexpr (e == B - 1 || k == NE - 1) {
+        if constexpr (e < B - 1)  // last, partial batch: clear the unused slots
+          static_for<e + 1, B>([&](auto ec) {
+            static_for<0, NC>([&](auto cc) { buf[decltype(cc)::value * B + decltype(ec)::value] = (T)0; });
+          });
+        transpose_sum<NV, G / 2>(buf, l);
+        tot[b] += buf[0];
       }
     });
+  }
+  // each lane's totals to LDS: lane l owns value l >> SHIFT of every batch
+  if ((l & ((1 << SHIFT) - 1)) == 0) {
+    constexpr EmitJju<J> ej{};
+    const int idx = l >> SHIFT, comp = idx / B, e = idx % B;
+    T* accT = reinterpret_cast<T*>(&acc[g][0][0]);
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+      const int k = b * B + e;
+      if (k < NE) {
+        const int jju = ej.v[k];
+        // S is (value) or (value, tangent); comp: 0 re, 1 im, 2 re tangent, 3 im tangent
+        accT[(jju * 2 + (comp & 1)) * (NC / 2) + (comp >> 1)] = tot[b];
+      }
+    }
   }
   __syncthreads();
   if (!valid) return;

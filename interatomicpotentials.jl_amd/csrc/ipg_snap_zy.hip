@@ -147,16 +147,12 @@ struct Plan {
 // its energy share and its Y contribution.
 template <int J, int j1, int j2, int j, int mb>
 __device__ __forceinline__ void triple(const float2* __restrict__ UL, const float* __restrict__ cgL, int h, int at,
-                                       const double* __restrict__ yc, const double* __restrict__ bt,
+                                       const double* __restrict__ ycs, const double* __restrict__ bts,
                                        const SnapDev& P, double* yr, double* yi, double& e) {
   constexpr int mb1lo = m1min(j1, j2, j, mb);
   constexpr int mb2hi = m2max(j1, j2, j, mb);
   constexpr int nb = mcount(j1, j2, j, mb);
   constexpr BlockPos bp = block_pos(J, j1, j2, j);
-  // Compiler-only fences keep the loads of each block in place: without
-  // them the row, coefficient and beta loads of every later block are
-  // hoisted above this loop and held in registers.
-  asm volatile("" ::: "memory");
   double zr[j + 1], zi[j + 1];
   static_for<0, j + 1>([&](auto mc) {
     zr[decltype(mc)::value] = 0.0;
@@ -185,7 +181,6 @@ __device__ __forceinline__ void triple(const float2* __restrict__ UL, const floa
       zi[ma] += cb * (double)si;
     });
   }
-  asm volatile("" ::: "memory");
   if constexpr (j >= j1) {
     // this half's share of B_(j1,j2,j) = 2 sum_half Re(conj(U_j) Z) (the
     // middle row of an even level counts its first half and half its centre)
@@ -202,9 +197,9 @@ __device__ __forceinline__ void triple(const float2* __restrict__ UL, const floa
     double ev = 2.0 * bs;
     if constexpr (mb == 0)
       if (h == 0 && P.bzero) ev -= P.bzero_j[j];
-    e += bt[bp.jjb] * ev;
+    e += bts[bp.jjb] * ev;
   }
-  const double cf = yc[bp.blk];
+  const double cf = ycs[bp.blk];
   static_for<0, j + 1>([&](auto mc) {
     yr[decltype(mc)::value] += cf * zr[decltype(mc)::value];
     yi[decltype(mc)::value] += cf * zi[decltype(mc)::value];
@@ -217,8 +212,8 @@ __device__ __forceinline__ void triple(const float2* __restrict__ UL, const floa
 
 template <int J, int j, int mb>
 __device__ __forceinline__ void unit(const float2* __restrict__ UL, const float* __restrict__ cgL, int h, int at,
-                                     const double* __restrict__ yc, const double* __restrict__ bt, const SnapDev& P,
-                                     double& e, float2* __restrict__ Yrow, bool store) {
+                                     const double* __restrict__ ycs, const double* __restrict__ bts,
+                                     const SnapDev& P, double& e, float2* __restrict__ Yrow, bool store) {
   double yr[j + 1], yi[j + 1];
   static_for<0, j + 1>([&](auto mc) {
     yr[decltype(mc)::value] = 0.0;
@@ -228,7 +223,7 @@ __device__ __forceinline__ void unit(const float2* __restrict__ UL, const float*
     constexpr int j1 = decltype(c1)::value;
     static_for<0, j1 + 1>([&](auto c2) {
       constexpr int j2 = decltype(c2)::value;
-      if constexpr (valid(J, j1, j2, j)) triple<J, j1, j2, j, mb>(UL, cgL, h, at, yc, bt, P, yr, yi, e);
+      if constexpr (valid(J, j1, j2, j)) triple<J, j1, j2, j, mb>(UL, cgL, h, at, ycs, bts, P, yr, yi, e);
     });
   });
   // the two halves hold partial sums over alternate mb1 rows
@@ -255,7 +250,6 @@ __device__ __forceinline__ void unit(const float2* __restrict__ UL, const float*
 template <int J, int NW>
 __global__ __launch_bounds__(64 * NW) void k_snap_zy32(SnapDev P, int64_t n, const int32_t* __restrict__ species,
                                                        const float2* __restrict__ Ub,
-                                                       const double* __restrict__ ycoef,
                                                        const double* __restrict__ beta, float2* __restrict__ Yb,
                                                        double* __restrict__ eatom) {
   constexpr int IDXU = ub_of(J + 1);
@@ -263,7 +257,9 @@ __global__ __launch_bounds__(64 * NW) void k_snap_zy32(SnapDev P, int64_t n, con
   extern __shared__ __align__(16) unsigned char smem_raw[];
   float2* UL = reinterpret_cast<float2*>(smem_raw);
   double* esum = reinterpret_cast<double*>(UL + IDXU * zy::AB);
-  float* cgL = reinterpret_cast<float*>(esum + NW * zy::AB);
+  double* ycL = esum + NW * zy::AB;                 // Y coefficients [nspecies][nblk]
+  double* btL = ycL + P.nspecies * P.nblk;          // beta [nspecies][ncoeff]
+  float* cgL = reinterpret_cast<float*>(btL + P.nspecies * P.ncoeff);
   const int tid = threadIdx.x;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int h = (tid >> 5) & 1, at = tid & 31;
@@ -273,17 +269,27 @@ __global__ __launch_bounds__(64 * NW) void k_snap_zy32(SnapDev P, int64_t n, con
   const float2* ug = Ub + batch * (int64_t)IDXU * zy::AB;
   for (int e = tid; e < IDXU * zy::AB; e += 64 * NW) UL[e] = ug[e];
   for (int e = tid; e < P.idxcg_max; e += 64 * NW) cgL[e] = P.cg_f[e];
+  for (int e = tid; e < P.nspecies * P.ncoeff; e += 64 * NW) btL[e] = beta[e];
+  // Y coefficient of each block and species: the beta entries whose dB/dr
+  // half-sums use the block, with their multiplicities
+  for (int e = tid; e < P.nspecies * P.nblk; e += 64 * NW) {
+    const int sp = e / P.nblk;
+    const YBlk* yb = P.yblk + e % P.nblk;
+    double c = 0.0;
+    for (int k = 0; k < yb->n; k++) c += beta[sp * P.ncoeff + yb->jjb[k]] * yb->fac[k];
+    ycL[e] = c;
+  }
   __syncthreads();
   const int s = valid && species ? species[a] : 0;
-  const double* yc = ycoef + (int64_t)s * P.nblk;
-  const double* bt = beta + (int64_t)s * P.ncoeff;
+  const double* ycs = ycL + s * P.nblk;
+  const double* bts = btL + s * P.ncoeff;
   float2* yb = Yb + batch * (int64_t)IDXU * zy::AB + at;
   double e = 0.0;
   static_for<0, zy::Plan<J, NW>::NU>([&](auto uc) {
     constexpr int u = decltype(uc)::value;
     constexpr int j = plan.uj[u], mb = plan.umb[u];
-    if (w == plan.owner[u]) zy::unit<J, j, mb>(UL, cgL, h, at, yc, bt, P, e, yb + (ub_of(j) + mb * (j + 1)) * zy::AB,
-                                                valid);
+    if (w == plan.owner[u])
+      zy::unit<J, j, mb>(UL, cgL, h, at, ycs, bts, P, e, yb + (ub_of(j) + mb * (j + 1)) * zy::AB, valid);
   });
   e += __shfl_xor(e, 32, 64);
   if (h == 0) esum[w * zy::AB + at] = e;
@@ -296,34 +302,34 @@ __global__ __launch_bounds__(64 * NW) void k_snap_zy32(SnapDev P, int64_t n, con
 }
 
 template <int J>
-static void launch_zy_j(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const float* U, const double* yc,
-                        const double* beta, float* Y, double* eatom) {
+static void launch_zy_j(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const float* U, const double* beta,
+                        float* Y, double* eatom) {
   constexpr int NU = zy::num_units(J);
   constexpr int NW = NU < 12 ? NU : 12;
   const size_t sm = sizeof(float2) * (size_t)ub_of(J + 1) * zy::AB + sizeof(double) * NW * zy::AB +
-                    sizeof(float) * (size_t)P.idxcg_max;
+                    sizeof(double) * (size_t)P.nspecies * (P.nblk + P.ncoeff) + sizeof(float) * (size_t)P.idxcg_max;
   const unsigned grid = (unsigned)((R.n + zy::AB - 1) / zy::AB);
   if (sm > 64 * 1024)
     IPG_HIP(hipFuncSetAttribute((const void*)k_snap_zy32<J, NW>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
-  k_snap_zy32<J, NW><<<grid, 64 * NW, sm, ctx->stream>>>(P, R.n, R.species, reinterpret_cast<const float2*>(U), yc,
-                                                         beta, reinterpret_cast<float2*>(Y), eatom);
+  k_snap_zy32<J, NW><<<grid, 64 * NW, sm, ctx->stream>>>(P, R.n, R.species, reinterpret_cast<const float2*>(U), beta,
+                                                         reinterpret_cast<float2*>(Y), eatom);
 }
 
-void snap_launch_zy32(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const float* U, const double* yc,
-                      const double* beta, float* Y, double* eatom) {
+void snap_launch_zy32(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const float* U, const double* beta, float* Y,
+                      double* eatom) {
   if (R.n == 0) return;
   IPG_REQUIRE(!P.bnorm, "internal: bnorm is not supported by the FP32 adjoint kernel");
   launch(ctx, "snap_zy_fp32", [&] {
     switch (P.J) {
-      case 0: launch_zy_j<0>(ctx, P, R, U, yc, beta, Y, eatom); break;
-      case 1: launch_zy_j<1>(ctx, P, R, U, yc, beta, Y, eatom); break;
-      case 2: launch_zy_j<2>(ctx, P, R, U, yc, beta, Y, eatom); break;
-      case 3: launch_zy_j<3>(ctx, P, R, U, yc, beta, Y, eatom); break;
-      case 4: launch_zy_j<4>(ctx, P, R, U, yc, beta, Y, eatom); break;
-      case 5: launch_zy_j<5>(ctx, P, R, U, yc, beta, Y, eatom); break;
-      case 6: launch_zy_j<6>(ctx, P, R, U, yc, beta, Y, eatom); break;
-      case 7: launch_zy_j<7>(ctx, P, R, U, yc, beta, Y, eatom); break;
-      default: launch_zy_j<8>(ctx, P, R, U, yc, beta, Y, eatom); break;
+      case 0: launch_zy_j<0>(ctx, P, R, U, beta, Y, eatom); break;
+      case 1: launch_zy_j<1>(ctx, P, R, U, beta, Y, eatom); break;
+      case 2: launch_zy_j<2>(ctx, P, R, U, beta, Y, eatom); break;
+      case 3: launch_zy_j<3>(ctx, P, R, U, beta, Y, eatom); break;
+      case 4: launch_zy_j<4>(ctx, P, R, U, beta, Y, eatom); break;
+      case 5: launch_zy_j<5>(ctx, P, R, U, beta, Y, eatom); break;
+      case 6: launch_zy_j<6>(ctx, P, R, U, beta, Y, eatom); break;
+      case 7: launch_zy_j<7>(ctx, P, R, U, beta, Y, eatom); break;
+      default: launch_zy_j<8>(ctx, P, R, U, beta, Y, eatom); break;
     }
   });
 }

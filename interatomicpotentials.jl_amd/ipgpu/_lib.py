@@ -32,6 +32,7 @@ EXPORTED = [
     "ipg_snap_create", "ipg_snap_destroy", "ipg_snap_info", "ipg_snap_neighborlist", "ipg_snap_local_descriptors",
     "ipg_snap_force_descriptors", "ipg_snap_all_descriptors", "ipg_snap_energy_forces",
     "ipg_snap_energy_forces_dev", "ipg_snap_force_vjp", "ipg_ctx_set_graphs", "ipg_snap_graph_info",
+    "ipg_device_copy",
 ]
 
 
@@ -125,6 +126,7 @@ def load():
             "ipg_snap_energy_forces_dev": (C.c_int, [P, pS, C.c_int, P, d, C.c_int, P, P]),
             "ipg_snap_force_vjp": (C.c_int, [P, pS, C.c_int, P, P, i64, P, C.c_int, P, P]),
             "ipg_ctx_set_graphs": (C.c_int, [P, C.c_int]),
+            "ipg_device_copy": (C.c_int, [P, P, P, C.c_size_t]),
             "ipg_snap_graph_info": (C.c_int, [P, C.POINTER(i32), C.POINTER(i64), C.POINTER(i64), C.POINTER(i64),
                                               C.c_char_p, i64]),
         }

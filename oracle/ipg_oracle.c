@@ -1096,6 +1096,48 @@ int orc_snap_all_descriptors(const orc_snap_params* p, const orc_snap_tables* t,
   return 0;
 }
 
+int orc_snap_plain_energy_forces(const orc_snap_params* p, const orc_snap_tables* t, int64_t n, int64_t nloop,
+                                 const int32_t* species, const int64_t* offsets, const int32_t* nbr,
+                                 const double* rvec, const double* beta, int nthreads, double* energy,
+                                 double* forces) {
+  int ncoeff = orc_snap_num_coeffs(p->twojmax, p->nspecies, p->chem_flag);
+  double E = 0.0;
+  memset(forces, 0, sizeof(double) * 3 * n);
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : E) num_threads(nthreads > 0 ? nthreads : 1)
+  for (int64_t i = 0; i < nloop; i++) {
+    double *rij, *wj, *rcutij;
+    int* elem;
+    int nnb = atom_pairs(p, i, species, offsets, nbr, rvec, &rij, &wj, &rcutij, &elem);
+    orc_rt rt;
+    rt_alloc(&rt, t, nnb);
+    int ie = species[i] + 1;
+    const double* bet = beta + (int64_t)ncoeff * (ie - 1);
+    compute_ui(p, &rt, ie, rij, wj, rcutij, elem);
+    compute_zi(p, &rt);
+    compute_bi(p, &rt);
+    for (int c = 0; c < ncoeff; c++) E += bet[c] * rt.blist[c];
+    for (int ind = 0; ind < nnb; ind++) {
+      int64_t jj = nbr[offsets[i] + ind];
+      compute_duidrj(p, t, rij + 3 * ind, wj[ind], rcutij[ind], rt.ulist_r + (ind + 1) * t->idxu_max,
+                     rt.ulist_i + (ind + 1) * t->idxu_max, rt.dulist_r, rt.dulist_i);
+      compute_dbidrj(p, &rt, elem[ind]);
+      double f[3] = {0.0, 0.0, 0.0};
+      for (int c = 0; c < ncoeff; c++)
+        for (int k = 0; k < 3; k++) f[k] += bet[c] * rt.dblist[3 * c + k];
+      for (int k = 0; k < 3; k++) {
+#pragma omp atomic
+        forces[3 * i + k] += f[k];
+#pragma omp atomic
+        forces[3 * jj + k] -= f[k];
+      }
+    }
+    rt_free(&rt);
+    free(rij); free(wj); free(rcutij); free(elem);
+  }
+  *energy = E;
+  return 0;
+}
+
 /* ------------------------------------------------------------------ */
 /* Adjoint (derived from the B formula; compute_yi.jl:6-122 and         */
 /* compute_dij.jl:5-74 are the specification only).                    */

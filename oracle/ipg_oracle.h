@@ -146,6 +146,17 @@ int orc_snap_local_descriptors(const orc_snap_params* p, const orc_snap_tables* 
 int orc_snap_all_descriptors(const orc_snap_params* p, const orc_snap_tables* t, int64_t n,
                              const double* pos, const int32_t* species, const int64_t* offsets,
                              const int32_t* nbr, const double* rvec, double* B, double* dB, double* W);
+/* Energy and forces of a linear SNAP potential by the reference's plain
+ * route (snap.jl:237-265 then linear_potential.jl:20-35): per atom U, Z, B
+ * and E_i = beta_s . B_i; per neighbor dB/dr_ij, f = beta_s . dB/dr_ij,
+ * F_i += f, F_j -= f, over the first nloop of the n atoms (forces has 3n
+ * entries).  nthreads > 1 splits the atoms over OpenMP threads (forces then
+ * accumulate with atomic adds, so their rounding depends on the schedule).
+ * Used to time the CPU baseline and checked against the adjoint. */
+int orc_snap_plain_energy_forces(const orc_snap_params* p, const orc_snap_tables* t, int64_t n, int64_t nloop,
+                                 const int32_t* species, const int64_t* offsets, const int32_t* nbr,
+                                 const double* rvec, const double* beta, int nthreads, double* energy,
+                                 double* forces);
 /* Energy and forces of a linear SNAP potential through the adjoint
  * Y = dE/dU (derived, compute_yi.jl is a specification only).  beta has
  * ncoeff*nspecies entries (block s is used for atoms of species s).

@@ -111,6 +111,9 @@ def lib():
         L.orc_snap_adjoint_energy_forces.restype = C.c_int
         L.orc_snap_adjoint_energy_forces.argtypes = [C.POINTER(SnapParams), C.POINTER(SnapTables), i64,
                                                      P, P, P, P, P, d, P, P, P, P]
+        L.orc_snap_plain_energy_forces.restype = C.c_int
+        L.orc_snap_plain_energy_forces.argtypes = [C.POINTER(SnapParams), C.POINTER(SnapTables), i64, i64,
+                                                   P, P, P, P, P, C.c_int, P, P]
         L.orc_snap_atom_uzb.restype = C.c_int
         L.orc_snap_atom_uzb.argtypes = [C.POINTER(SnapParams), C.POINTER(SnapTables), C.c_int, C.c_int,
                                         P, P, P, P, P, P]
@@ -295,6 +298,24 @@ class Snap:
         lib().orc_snap_all_descriptors(C.byref(self.p), self.t, n, _p(pos), _p(sp), _p(off), _p(nbr), _p(rv),
                                        _p(B), _p(dB), _p(W))
         return B, dB, W
+
+    def plain_energy_forces(self, species, off, nbr, rvec, beta, nthreads=1, count=None):
+        """Reference-route E, F (B and per-pair dB contracted with beta) over
+        the first `count` atoms (all by default); nthreads > 1 uses OpenMP.
+        Returns (energy, forces, seconds)."""
+        n = len(species)
+        count = n if count is None else int(count)
+        sp = np.ascontiguousarray(species, dtype=np.int32)
+        beta = np.ascontiguousarray(beta, dtype=np.float64).reshape(-1)
+        if beta.size == self.ncoeff and self.nspecies > 1:
+            beta = np.tile(beta, self.nspecies)
+        e = np.zeros(1)
+        F = np.zeros((n, 3))
+        rv = np.ascontiguousarray(rvec, dtype=np.float64)
+        t0 = lib().orc_wtime()
+        lib().orc_snap_plain_energy_forces(C.byref(self.p), self.t, n, count, _p(sp), _p(off), _p(nbr), _p(rv),
+                                           _p(beta), int(nthreads), _p(e), _p(F))
+        return e[0], F, lib().orc_wtime() - t0
 
     def adjoint_energy_forces(self, species, off, nbr, rvec, beta, beta0=0.0, want_y=False, want_dedr=False):
         n = len(species)

@@ -362,3 +362,28 @@ def test_adjoint_equals_plain_and_fd_energy(orc, switch):
             o2, n2, _, r2 = s.neighborlist(Xc, L, sp, mode=orc.SNAP_LIST_CUTOFF)
             Ef.append(s.adjoint_energy_forces(sp, o2, n2, r2, beta)["energy"])
         assert -(Ef[0] - Ef[1]) / (2 * h) == pytest.approx(ad["forces"][a, k], rel=1e-6, abs=1e-8)
+
+
+@pytest.mark.parametrize("nthreads", [1, 3])
+def test_plain_energy_forces_equals_descriptors(orc, nthreads):
+    """The CPU-baseline routine (plain route, optionally OpenMP) against the
+    descriptor route beta . dB and beta . B, over all atoms and over a
+    leading subset of atoms."""
+    s, X, L, sp = _bcc(orc, 5, True)
+    off, nb, R, r = s.neighborlist(X, L, sp, mode=orc.SNAP_LIST_CUTOFF)
+    beta = np.random.default_rng(2).normal(scale=0.1, size=2 * s.ncoeff)
+    B, dB, _ = s.all_descriptors(X, sp, off, nb, r)
+    Fp = np.einsum("nmk,m->nk", dB, beta)
+    Ep = sum(beta[sp[i] * s.ncoeff:(sp[i] + 1) * s.ncoeff] @ B[i] for i in range(len(X)))
+    e, F, _ = s.plain_energy_forces(sp, off, nb, r, beta, nthreads=nthreads)
+    assert e == pytest.approx(Ep, rel=1e-12)
+    assert np.abs(F - Fp).max() < 1e-12 * np.abs(Fp).max()
+    # leading subset: the pairs centred on the first k atoms, i.e. the
+    # adjoint oracle on the same list with the rows of atoms k.. emptied
+    k = 40
+    e_k, F_k, _ = s.plain_energy_forces(sp, off, nb, r, beta, nthreads=nthreads, count=k)
+    assert e_k == pytest.approx(sum(beta[sp[i] * s.ncoeff:(sp[i] + 1) * s.ncoeff] @ B[i] for i in range(k)),
+                                rel=1e-12)
+    off_k = np.minimum(off, off[k])
+    ad = s.adjoint_energy_forces(sp, off_k, nb, r, beta)
+    assert np.abs(F_k - ad["forces"]).max() < 1e-12 * np.abs(ad["forces"]).max()
