@@ -165,7 +165,9 @@ template <typename S>
 void snap_launch_dedr(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const S* Y, double* dedr, double* dedr_t);
 template <typename T>
 void snap_launch_ycoef(ipg_ctx* ctx, const SnapDev& P, const double* beta, T* ycoef);
-void snap_launch_dbdr_plain(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const double* Z, double* dblist);
+// Per-pair dB_i/dr_ij [P][ncoeff][3] from the center atoms' Z [N][idxz_max][2].
+template <typename S>
+void snap_launch_dbdr(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const S* Z, S* dblist);
 // FP32 adjoint Z -> (energy per atom, Y) on the atom-blocked U/Y layout
 // [ceil(n/32)][idxu_max][32][2]; beta expanded [nspecies][ncoeff].
 void snap_launch_zy32(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const float* U, const double* beta, float* Y,

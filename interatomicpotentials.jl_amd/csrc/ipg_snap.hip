@@ -334,24 +334,35 @@ static void force_gather(ipg_ctx* ctx, const SnapRun& R, const double* dedr, dou
   });
 }
 
-// dB[a] (M x 3, blocks by the species of the pair's center atom).
+// dB[a] (M x 3, blocks by the species of the pair's center atom,
+// snap.jl:264-265): block s(a) gets the atom's own pairs, block s gets minus
+// the reverse pairs whose center has species s.  One pass over the own row
+// and one over the reverse row, each in ascending pair order.
 __global__ void k_db_gather(SnapDev P, SnapRun R, const double* __restrict__ dblist, double* __restrict__ dB) {
   const int64_t a = blockIdx.x;
   const int M = P.ncoeff * P.nspecies;
   const int sa = R.species ? R.species[a] : 0;
+  const int64_t q0 = R.off[a], q1 = R.off[a + 1], r0 = R.roff[a], r1 = R.roff[a + 1];
   for (int t = threadIdx.x; t < 3 * P.ncoeff; t += blockDim.x) {
-    const int c = t / 3, k = t % 3;
-    for (int s = 0; s < P.nspecies; s++) {
-      double acc = 0.0;
-      if (s == sa)
-        for (int64_t q = R.off[a]; q < R.off[a + 1]; q++) acc += dblist[(q * P.ncoeff + c) * 3 + k];
-      for (int64_t r = R.roff[a]; r < R.roff[a + 1]; r++) {
-        const int64_t q = R.rpair[r];
-        const int sc = R.species ? R.species[R.center[q]] : 0;
-        if (sc == s) acc -= dblist[(q * P.ncoeff + c) * 3 + k];
-      }
-      dB[(a * M + s * P.ncoeff + c) * 3 + k] = acc;
+    double acc[8];
+#pragma unroll
+    for (int s = 0; s < 8; s++) acc[s] = 0.0;
+    double own = 0.0;
+    for (int64_t q = q0; q < q1; q++) own += dblist[q * 3 * P.ncoeff + t];
+#pragma unroll
+    for (int s = 0; s < 8; s++)
+      if (s == sa) acc[s] = own;
+    for (int64_t r = r0; r < r1; r++) {
+      const int64_t q = R.rpair[r];
+      const int sc = R.species ? R.species[R.center[q]] : 0;
+      const double v = dblist[q * 3 * P.ncoeff + t];
+#pragma unroll
+      for (int s = 0; s < 8; s++)
+        if (s == sc) acc[s] -= v;
     }
+#pragma unroll
+    for (int s = 0; s < 8; s++)
+      if (s < P.nspecies) dB[(a * M + s * P.ncoeff) * 3 + t] = acc[s];
   }
 }
 
@@ -510,7 +521,7 @@ static double* run_descriptors_fp64(ipg_snap* sn, const SnapCall& c, double* dB_
 static double* run_dblist(ipg_snap* sn, const SnapCall& c, const double* Z) {
   const SnapDev& P = sn->dev;
   double* dbl = scratch_t<double>(sn->ctx, "snap_dblist", 3 * (size_t)P.ncoeff * std::max<int64_t>(c.R.npairs, 1));
-  snap_launch_dbdr_plain(sn->ctx, P, c.R, Z, dbl);
+  snap_launch_dbdr<double>(sn->ctx, P, c.R, Z, dbl);
   return dbl;
 }
 
