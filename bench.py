@@ -103,6 +103,87 @@ def cpu_baseline_snap(s, beta, sp, sample1):
     return {"threads": nth, "rate": n / tn, "seconds": tn, "rate1": k / t1, "seconds1": t1, "sample1": k}
 
 
+def batch_bench(steps, warmup, nconfig, cpu_configs):
+    """Batched SNAP 2J=8 descriptors (B, dB and per-configuration energy and
+    virial rows) of `nconfig` periodic configurations with 50..2000 atoms each, in
+    one call per step: FP32 (FP64 sums) and FP64, each replayed from its
+    captured HIP graph, plus one uncaptured FP32 pass with per-kernel
+    events.  CPU: the oracle's descriptors of the first `cpu_configs`
+    configurations on one thread."""
+    import ipgpu
+    from ipgpu import workloads
+    from ipgpu._lib import IPG_FP32, IPG_FP64, check
+    from ipgpu.device import DeviceArray, prof_enable, prof_reset, set_graphs, synchronize
+
+    names = ["W", "Mo"]
+    systems = workloads.snap_batch(nconfig=nconfig, nmin=50, nmax=2000, rcut=5.0, seed=4)
+    sn = ipgpu.snap.SNAP(100, 8, names, 5.0, 0.0, 0.99363, [0.5, 0.5], [1.0, 0.9], switch_flag=True,
+                         list_mode="cutoff")
+    natoms = np.array([len(s) for s in systems], dtype=np.int64)
+    n = int(natoms.sum())
+    M = 2 * sn.ncoeff
+    box = np.array([s.box for s in systems], dtype=np.float64)
+    per = np.ones((nconfig, 3), dtype=np.int32)
+    pos = DeviceArray.from_host(np.concatenate([s.positions for s in systems]))
+    sp = DeviceArray.from_host(np.concatenate([[names.index(x) for x in s.symbols] for s in systems]).astype(np.int32))
+    B = DeviceArray((n, sn.ncoeff), np.float64)
+    dB = DeviceArray((n, M, 3), np.float64)
+    er = DeviceArray((nconfig, M), np.float64)
+    W = DeviceArray((nconfig, M, 6), np.float64)
+    L = ipgpu.load()
+
+    def step(prec):
+        check(L.ipg_snap_batch_descriptors_dev(sn._h, nconfig, natoms.ctypes.data, box.ctypes.data, per.ctypes.data,
+                                               C.c_void_p(pos.ptr), C.c_void_p(sp.ptr), prec, C.c_void_p(B.ptr),
+                                               C.c_void_p(dB.ptr), C.c_void_p(er.ptr), C.c_void_p(W.ptr)))
+
+    res = {"configs": nconfig, "natoms": n, "twojmax": 8, "outputs": "B, dB, energy rows, virial rows"}
+    for name, prec in (("fp32", IPG_FP32), ("fp64", IPG_FP64)):
+        t0 = time.perf_counter()
+        step(prec)  # first call: autotuning, exact run, capture
+        synchronize()
+        first = time.perf_counter() - t0
+        t = time_steps(lambda: step(prec), steps, warmup)
+        res[name] = {"ms_per_batch": t * 1e3, "atoms_per_s": n / t, "configs_per_s": nconfig / t,
+                     "first_call_s": first, "tuning": ipgpu.snap.batch_info(sn, name)}
+        log("batch %s: %.2f ms per batch of %d configs (%d atoms)" % (name, t * 1e3, nconfig, n))
+        if name == "fp32":
+            B32, er32 = B.download(), er.download()
+            dB32 = dB.download()
+    B64, er64, dB64 = B.download(), er.download(), dB.download()
+    rel = lambda a, b: float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+    res["fp32_vs_fp64"] = {"B": rel(B32, B64), "dB": rel(dB32, dB64), "erow": rel(er32, er64)}
+    del dB32, dB64
+    # per-kernel times of the FP32 pipeline, launched kernel by kernel
+    set_graphs(False)
+    prof_reset()
+    prof_enable(True)
+    step(IPG_FP32)
+    synchronize()
+    prof_enable(False)
+    set_graphs(True)
+    res["kernels_fp32_ms"] = {k: v[1] for k, v in kernel_avg(
+        ["batch_scan", "batch_list_finish", "snap_ui_fp32", "snap_zb_fp32", "snap_dbdr_fp32", "snap_db_gather",
+         "batch_erow", "batch_w_atom", "batch_wrow", "scan", "fill_zero"]).items()}
+    if cpu_configs > 0:
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import oracle as O
+
+        o = O.Snap(8, 2, 5.0, 0.0, 0.99363, [0.5, 0.5], [1.0, 0.9], switch_flag=True)
+        t0 = time.perf_counter()
+        na = 0
+        for s in systems[:cpu_configs]:
+            spc = np.array([names.index(x) for x in s.symbols], np.int32)
+            off, nbr, R, r = o.neighborlist(s.positions, s.box, spc, mode=O.SNAP_LIST_CUTOFF)
+            o.all_descriptors(s.positions, spc, off, nbr, r)
+            na += len(s)
+        tc = time.perf_counter() - t0
+        res["cpu_baseline"] = {"atoms_per_s": na / tc, "cores": 1, "kind": "port",
+                               "sample": "oracle B + dB of the first %d configurations (%d atoms), 1 thread, "
+                                         "%.2f s" % (cpu_configs, na, tc)}
+    return res
+
+
 def lj_bench(steps, warmup):
     import ipgpu
     from ipgpu import workloads, perfmodel
@@ -160,6 +241,22 @@ def lj_bench(steps, warmup):
                          "algorithmic_bytes": b},
         }
     check(L.ipg_nlist_destroy(lst))
+    # MD-style step over a Verlet-skin list (skin 0.5 A, positions unchanged,
+    # so no rebuild): displacement check + FP64 evaluation, against the
+    # reference's rebuild on every call (end_to_end above)
+    skin = C.c_void_p()
+    check(L.ipg_nlist_build_skin_dev(ctx, C.byref(dsys), pot.get_rcutoff(), 0.5 * ipgpu.angstrom, 1,
+                                     C.byref(skin)))
+
+    def md_step():
+        check(L.ipg_nlist_update_dev(skin, C.byref(dsys), None))
+        check(L.ipg_pair_eval_dev(ctx, skin, C.byref(dsys), C.cast(arr, C.c_void_p), len(terms), 0,
+                                  C.c_void_p(e.ptr), C.c_void_p(f.ptr), C.c_void_p(v.ptr)))
+
+    t_md = time_steps(md_step, max(3, steps // 2), 1)
+    res["fp64"]["md_step_ms_skin_list"] = t_md * 1e3
+    res["fp64"]["pairs_per_s_md_step_skin_list"] = half / t_md
+    check(L.ipg_nlist_destroy(skin))
     return res
 
 
@@ -213,6 +310,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-lj", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-configs", type=int, default=512, help="configurations of the batched descriptor run")
     ap.add_argument("--cpu-sample", type=int, default=2048, help="atoms of the 1-thread CPU baseline")
     ap.add_argument("--probe", action="store_true",
                     help="short run for hardware-counter passes: 3 calibration copies of 256 MiB, then 3 uncaptured "
@@ -322,6 +421,9 @@ def main():
     if not a.no_lj:
         log("LJ/Morse/ZBL 256k pair kernel")
         out["lj"] = lj_bench(max(5, a.steps), a.warmup)
+    if not a.no_batch:
+        log("batched SNAP descriptors")
+        out["batch"] = batch_bench(max(3, a.steps // 4), 1, a.batch_configs, 0 if a.no_cpu else 2)
     if not a.no_cpu:
         log("CPU baseline (oracle)")
         cb = cpu_baseline_snap(s, beta, sp, a.cpu_sample)

@@ -70,8 +70,13 @@ int ipg_prof_reset(ipg_ctx* ctx);
 /* Accumulated launches and milliseconds of the kernel named `kernel`
  * (flushes pending events, synchronizing the stream). */
 int ipg_prof_query(ipg_ctx* ctx, const char* kernel, int64_t* launches, double* total_ms);
+/* Restrict the timing to the kernels named in `names` (comma-separated;
+ * NULL or "" times every kernel).  Fewer event pairs disturb a run less,
+ * inside a captured graph in particular. */
+int ipg_prof_select(ipg_ctx* ctx, const char* names);
 /* Enable (default) or disable the replay of captured HIP graphs by the
- * entry points that support it (ipg_snap_energy_forces*).  With graphs off
+ * entry points that support it (ipg_snap_energy_forces*,
+ * ipg_snap_batch_descriptors*).  With graphs off
  * every call launches its kernels one by one.  No counterpart in the
  * reference (a launch-overhead control of this implementation). */
 int ipg_ctx_set_graphs(ipg_ctx* ctx, int on);
@@ -106,6 +111,20 @@ int ipg_nlist_build_dev(ipg_ctx* ctx, const ipg_system* sys, double rcut, int ki
 int ipg_nlist_info(const ipg_nlist* list, int64_t* natoms, int64_t* npairs, const int64_t** d_offsets,
                    const int32_t** d_nbr);
 int ipg_nlist_destroy(ipg_nlist* list);
+/* Verlet-skin list for MD callers, which the reference serves by rebuilding
+ * the list on every call (src/types/empirical_potential.jl:21): the list
+ * holds every pair within rcut + skin of the positions at its last build.
+ * ipg_nlist_update_dev measures the largest PeriodicEuclidean displacement
+ * of any atom since then (one synchronization) and rebuilds when it exceeds
+ * skin/2 or the box changed, so every pair within rcut of the current
+ * positions is in the list; *rebuilt (may be NULL) says whether it rebuilt.
+ * ipg_pair_eval_dev over such a list tests each pair's membership against
+ * the terms' own cutoffs with the reference's distance formula, so its
+ * result equals a fresh list's up to the summation order.  No counterpart
+ * in the reference. */
+int ipg_nlist_build_skin_dev(ipg_ctx* ctx, const ipg_system* sys, double rcut, double skin, int kind,
+                             ipg_nlist** out);
+int ipg_nlist_update_dev(ipg_nlist* list, const ipg_system* sys, int32_t* rebuilt);
 
 /* ------------------------------------------------------------------ */
 /* pair potentials                                                     */
@@ -208,9 +227,11 @@ int ipg_snap_energy_forces(ipg_snap* snap, const ipg_system* sys, int mode, cons
  * step is captured into a HIP graph on the first call and replayed while
  * the same buffers, box, mode, path and beta0 are passed (an MD loop);
  * neighbor lists then run at capacities sized from the first call's pair
- * counts and each call synchronizes once to read a capacity-overflow flag
- * (an overflowing call is recomputed with exact sizes).  Results equal the
- * uncaptured call's bit for bit. */
+ * counts (an overflowing call is recomputed with exact sizes).  Every call
+ * synchronizes the context stream once, at its end, to read the list's
+ * flag word (capacity overflow, and in CUTOFF mode a pair whose reverse pair
+ * is missing, which is then fixed up), so the outputs are complete when it
+ * returns.  Results equal the uncaptured call's bit for bit. */
 int ipg_snap_energy_forces_dev(ipg_snap* snap, const ipg_system* sys, int mode, const double* beta, double beta0,
                                int path, double* energy, double* forces);
 /* Captured-step statistics of a snap object: state 0 = no graph, 1 = graph
@@ -220,6 +241,42 @@ int ipg_snap_energy_forces_dev(ipg_snap* snap, const ipg_system* sys, int mode, 
  * in the reference. */
 int ipg_snap_graph_info(const ipg_snap* snap, int32_t* state, int64_t* captures, int64_t* replays,
                         int64_t* overflows, char* reason, int64_t reason_len);
+/* Descriptors of many configurations in one call: replaces a fitting loop
+ * that maps compute_local_descriptors and compute_force_descriptors
+ * (src/BasisSystems/SNAP/snap.jl:103-124, :126-224, computed as
+ * compute_all_descriptors does, :226-285; generic functions at
+ * src/api.jl:151-173) over a data set, in CUTOFF list mode.  systems[c]
+ * are nconf host-pointer systems.  Outputs are packed in configuration
+ * order, atoms of configuration c following those of c-1:
+ * B [natoms_total][ncoeff], dB [natoms_total][ncoeff*nspecies][3] (the
+ * Julia 3 x M matrix of each atom), erow [nconf][nspecies][ncoeff], the
+ * per-configuration sums of B in species blocks (the row the energy of
+ * linear_potential.jl:20-26 contracts with beta), and wrow
+ * [nconf][ncoeff*nspecies][6], each configuration's W of
+ * compute_all_descriptors (snap.jl:267-281, raw positions, rows
+ * [x dBx, y dBy, z dBz, z dBy, z dBx, y dBx]).  Any output may be NULL.
+ * precision IPG_FP64 runs the FP64 kernels; IPG_FP32 computes U, Z and the
+ * per-pair dB in FP32 with FP64 sums for B, dB and erow.  The first call of
+ * each precision times candidate launch shapes on the batch (autotuning);
+ * later calls with the same layout replay one captured HIP graph. */
+int ipg_snap_batch_descriptors(ipg_snap* snap, int64_t nconf, const ipg_system* systems, int precision, double* B,
+                               double* dB, double* erow, double* wrow);
+/* Device variant: the layout (natoms[nconf], box[nconf][3],
+ * periodic[nconf][3]) is on the host; pos (3*natoms_total), species
+ * (natoms_total, may be NULL) and the outputs are device pointers in the
+ * packed order above.  Species indices are not checked on the device.  The
+ * call synchronizes the context stream once at its end (it reads the
+ * neighbor-list flag word and recomputes exactly on a capacity overflow). */
+int ipg_snap_batch_descriptors_dev(ipg_snap* snap, int64_t nconf, const int64_t* natoms, const double* box,
+                                   const int32_t* periodic, const double* pos, const int32_t* species, int precision,
+                                   double* B, double* dB, double* erow, double* wrow);
+/* Launch shapes the autotuner chose for a precision (0 before the first
+ * call): lanes per atom of the neighbor scan, atoms per wave of the U
+ * kernel and block size of the per-pair dB kernel; graph_state as in
+ * ipg_snap_graph_info, and the graph replays so far.  No counterpart in the
+ * reference. */
+int ipg_snap_batch_info(const ipg_snap* snap, int precision, int32_t* scan_lanes, int32_t* ui_atoms_per_wave,
+                        int32_t* dbdr_threads, int32_t* graph_state, int64_t* replays);
 /* Reverse mode of F(x, beta) for force matching (a ChainRulesCore rrule of
  * the force call): given the cotangent gF = dL/dF (3*natoms), returns
  * gbeta = gF^T dF/dbeta (nbeta entries, same layout as beta) and

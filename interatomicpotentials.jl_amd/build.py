@@ -36,11 +36,30 @@ CFLAGS = [
 ]
 
 
+def _includes(path, seen=None):
+    """Local headers a source includes, followed recursively (quoted
+    #include lines resolved against csrc/ and include/)."""
+    seen = set() if seen is None else seen
+    with open(path) as f:
+        for line in f:
+            line = line.strip()
+            if not line.startswith("#include \""):
+                continue
+            name = line.split('"')[1]
+            for d in (CSRC, os.path.join(REPO, "include")):
+                h = os.path.join(d, name)
+                if os.path.exists(h) and h not in seen:
+                    seen.add(h)
+                    _includes(h, seen)
+                    break
+    return seen
+
+
 def _stale(src, obj, headers):
     if not os.path.exists(obj):
         return True
     t = os.path.getmtime(obj)
-    return any(os.path.getmtime(f) > t for f in [src] + headers)
+    return any(os.path.getmtime(f) > t for f in [src] + sorted(_includes(src)))
 
 
 def _compile(src, verbose, incremental=False, headers=()):

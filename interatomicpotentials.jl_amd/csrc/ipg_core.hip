@@ -482,6 +482,27 @@ int ipg_prof_enable(ipg_ctx* ctx, int on) {
     IPG_REQUIRE(ctx, "ctx is NULL");
     flush_prof(ctx);
     ctx->prof = on != 0;
+    ctx->prof_gen++;
+  });
+}
+
+int ipg_prof_select(ipg_ctx* ctx, const char* names) {
+  return guard([&] {
+    IPG_REQUIRE(ctx, "ctx is NULL");
+    flush_prof(ctx);
+    ctx->prof_only.clear();
+    if (names) {
+      std::string all(names), cur;
+      for (char ch : all + ",") {
+        if (ch == ',') {
+          if (!cur.empty()) ctx->prof_only.push_back(cur);
+          cur.clear();
+        } else if (ch != ' ') {
+          cur += ch;
+        }
+      }
+    }
+    ctx->prof_gen++;
   });
 }
 

@@ -86,6 +86,8 @@ struct ipg_ctx {
   bool graphs = true;        // replay captured HIP graphs where a path supports it
   bool dry = false;          // allocation-only pass: launches are skipped
   std::vector<Pending>* capture_events = nullptr;  // event pairs of a graph being captured
+  std::vector<std::string> prof_only;  // kernels timed when profiling (empty: all)
+  uint64_t prof_gen = 0;               // bumped when the profiling setup changes
 };
 
 namespace ipg {
@@ -110,18 +112,26 @@ inline void record_event(ipg_ctx* ctx, hipEvent_t e) {
   else IPG_HIP(hipEventRecord(e, ctx->stream));
 }
 
+inline bool prof_selected(const ipg_ctx* ctx, const char* name) {
+  if (ctx->prof_only.empty()) return true;
+  for (const std::string& k : ctx->prof_only)
+    if (k == name) return true;
+  return false;
+}
+
 template <typename L>
 void launch(ipg_ctx* ctx, const char* name, L&& l) {
   if (ctx->dry) return;
   hipEvent_t a = nullptr, b = nullptr;
-  if (ctx->prof) {
+  const bool timed = ctx->prof && prof_selected(ctx, name);
+  if (timed) {
     a = get_event(ctx);
     b = get_event(ctx);
     record_event(ctx, a);
   }
   l();
   IPG_HIP(hipGetLastError());
-  if (ctx->prof) {
+  if (timed) {
     record_event(ctx, b);
     (ctx->capture_events ? *ctx->capture_events : ctx->pending).push_back({name, a, b});
   }

@@ -74,6 +74,23 @@ SnapListDev build_snap_list(ipg_ctx* ctx, const DevSystem& s, const int32_t* d_a
                             const char* tag, int64_t capA = 0, int64_t capB = 0, int32_t* flag = nullptr,
                             int ellK = 0);
 
+// Packed configurations on the device: atoms of configuration c are
+// start[c] .. start[c+1]-1, cfg[i] is the configuration of atom i, boxes[c]
+// its box; expect is the largest expected neighbor count (row width hint).
+struct BatchSys {
+  int64_t n = 0, nconf = 0;
+  const double* pos = nullptr;
+  const int32_t* species = nullptr;
+  const int32_t* cfg = nullptr;
+  const int64_t* start = nullptr;
+  const IpgBox* boxes = nullptr;
+  double expect = 0.0;
+};
+// CUTOFF-mode SNAP list of packed configurations (rows sorted by j).  cap,
+// flag and ellK as build_snap_list; lp: lanes per atom (16, 32 or 64).
+SnapListDev build_snap_list_batch(ipg_ctx* ctx, const BatchSys& b, const SnapCut& cut, int need, const char* tag,
+                                  int64_t cap, int32_t* flag, int ellK, int lp);
+
 // center[q] and the reverse list of any CSR list (npairs may be a
 // capacity: the count used is off[n] on the device).
 void build_center_and_reverse(ipg_ctx* ctx, int64_t n, int64_t npairs, const int64_t* off, const int32_t* nbr,
@@ -84,6 +101,12 @@ void build_center_and_reverse(ipg_ctx* ctx, int64_t n, int64_t npairs, const int
 struct ipg_nlist {
   ipg_ctx* ctx = nullptr;
   ipg::DevList list;
-  double rcut = 0;
+  double rcut = 0;  // radius the list was built at (cut + skin)
+  double cut = 0;   // interaction cutoff the list serves
+  double skin = 0;  // Verlet skin (0: a plain list)
   int kind = 0;
+  double* ref = nullptr;  // positions at the last build (skin lists; device)
+  IpgBox box{};           // box of the last build
+  uint64_t* dmax = nullptr;  // device word: largest squared displacement (as bits)
+  int64_t builds = 0, updates = 0;
 };

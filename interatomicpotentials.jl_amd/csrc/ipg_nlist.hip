@@ -223,7 +223,7 @@ __global__ __launch_bounds__(256) void k_list_scan(int64_t n, Grid g, Prefilter 
   const int l = (int)(gt % LP);
   const int lane = threadIdx.x & 63;
   if (k >= n) return;  // whole groups leave together
-  const uint64_t gmask = (LP == 64 ? ~0ull : ((1ull << LP) - 1ull)) << (lane - l);
+  const uint64_t gmask = (LP == 64 ? ~0ull : ((1ull << (LP % 64)) - 1ull)) << (lane - l);
   const uint64_t below = (1ull << lane) - 1ull;
   const int32_t i = catom[k];
   const double4 pi = spos[k];
@@ -736,7 +736,8 @@ template <int LP>
 __global__ __launch_bounds__(256) void k_snapc_finish(int64_t n, int K, const int32_t* __restrict__ ell,
                                                       const int32_t* __restrict__ cnt,
                                                       const int64_t* __restrict__ off, const double* __restrict__ pos,
-                                                      IpgBox box, int32_t* __restrict__ nbr,
+                                                      IpgBox box1, const int32_t* __restrict__ cfg,
+                                                      const IpgBox* __restrict__ boxes, int32_t* __restrict__ nbr,
                                                       int32_t* __restrict__ center, double* __restrict__ rvec,
                                                       double* __restrict__ R, int32_t* __restrict__ rev,
                                                       int32_t* __restrict__ flag) {
@@ -757,6 +758,7 @@ __global__ __launch_bounds__(256) void k_snapc_finish(int64_t n, int K, const in
   }
   __syncthreads();
   if (!act) return;
+  const IpgBox box = cfg ? boxes[cfg[i]] : box1;  // packed configurations: the atom's own box
   const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
   for (int k = l; k < c; k += LP) {
     const int32_t j = row[k];
@@ -828,10 +830,12 @@ SnapListDev build_snap_list(ipg_ctx* ctx, const DevSystem& s, const int32_t* d_a
       launch(ctx, "snap_list_finish", [&] {
         if (LP == 64)
           k_snapc_finish<64><<<grid_for(n * 64, 256), 256, sm, ctx->stream>>>(
-              n, K, L.ell, L.cnt, L.off, s.pos, s.box, L.nbr, out.center, out.rvec, out.R, out.rev, flag);
+              n, K, L.ell, L.cnt, L.off, s.pos, s.box, nullptr, nullptr, L.nbr, out.center, out.rvec, out.R,
+              out.rev, flag);
         else
           k_snapc_finish<32><<<grid_for(n * 32, 256), 256, sm, ctx->stream>>>(
-              n, K, L.ell, L.cnt, L.off, s.pos, s.box, L.nbr, out.center, out.rvec, out.R, out.rev, flag);
+              n, K, L.ell, L.cnt, L.off, s.pos, s.box, nullptr, nullptr, L.nbr, out.center, out.rvec, out.R,
+              out.rev, flag);
       });
     } else if (n > 0 && L.npairs > 0) {
       launch(ctx, "snap_pairs", [&] {
@@ -891,6 +895,181 @@ SnapListDev build_snap_list(ipg_ctx* ctx, const DevSystem& s, const int32_t* d_a
   return out;
 }
 
+// ---------------------------------------------------------------------------
+// Packed configurations (batched descriptors): CUTOFF-mode SNAP lists of
+// many small systems in one pass.  Atom i of configuration c = cfg[i]
+// scans the atoms of c (start[c] .. start[c+1]) in index order with c's own
+// box, applying the two membership tests of the single-system list
+// (PeriodicEuclidean distance against the query radius, then the
+// species-pair cutoff on get_displacement, utl.jl:69-90); its hits land in
+// its row of `ell` in ascending j.
+// ---------------------------------------------------------------------------
+template <int LP>
+__global__ __launch_bounds__(256) void k_batch_scan(int64_t n, const int32_t* __restrict__ cfg,
+                                                    const int64_t* __restrict__ start,
+                                                    const IpgBox* __restrict__ boxes, const double* __restrict__ pos,
+                                                    const int32_t* __restrict__ species, double smax, SnapCutDev sc,
+                                                    int32_t* __restrict__ cnt, int32_t* __restrict__ ell, int K,
+                                                    int32_t* __restrict__ flag) {
+  const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = gt / LP;
+  const int l = (int)(gt % LP);
+  const int lane = threadIdx.x & 63;
+  if (i >= n) return;  // whole groups leave together
+  const uint64_t gmask = (LP == 64 ? ~0ull : ((1ull << LP) - 1ull)) << (lane - l);
+  const uint64_t below = (1ull << lane) - 1ull;
+  const int c = cfg[i];
+  const IpgBox box = boxes[c];
+  const int64_t lo = start[c], hi = start[c + 1];
+  const double4 pi = make_double4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], species ? (double)species[i] : 0.0);
+  int count = 0;
+  for (int64_t s0 = lo; s0 < hi; s0 += LP) {
+    const int64_t j = s0 + l;
+    bool hit = false;
+    if (j < hi && j != i) {
+      const double4 pj =
+          make_double4(pos[3 * j], pos[3 * j + 1], pos[3 * j + 2], species ? (double)species[j] : 0.0);
+      hit = ipg_pe_sumsq(box, pi.x, pi.y, pi.z, pj.x, pj.y, pj.z) <= smax && snap_cut_ok(box, pi, pj, sc);
+    }
+    const uint64_t m = __ballot(hit) & gmask;
+    const int r = count + __popcll(m & below);
+    if (hit && r < K) ell[i * K + r] = (int32_t)j;
+    count += __popcll(m);
+  }
+  if (l == 0) {
+    if (count > K) {
+      atomicOr(flag, 8);
+      count = K;  // the row holds what was written, so every list slot gets an entry
+    }
+    cnt[i] = count;
+  }
+}
+
+SnapListDev build_snap_list_batch(ipg_ctx* ctx, const BatchSys& b, const SnapCut& cut, int need, const char* tag,
+                                  int64_t cap, int32_t* flag, int ellK, int lp) {
+  IPG_REQUIRE(!cut.reference, "batched SNAP lists use CUTOFF mode");
+  IPG_REQUIRE(flag != nullptr, "internal: batched list without a flag word");
+  std::string t(tag);
+  SnapListDev out;
+  const int64_t n = b.n;
+  SnapCutDev cd{};
+  cd.ns = cut.ns;
+  for (int k = 0; k < cut.ns * cut.ns; k++) cd.cutsq[k] = cut.cutsq[k];
+  const double smax = sqrt_le_threshold(cut.query);
+  DevList& L = out.list;
+  L.n = n;
+  L.off = scratch_t<int64_t>(ctx, (t + "_off").c_str(), n + 1);
+  int32_t* cnt = scratch_t<int32_t>(ctx, (t + "_cnt").c_str(), n > 0 ? n : 1);
+  int K = ellK > 0 ? ellK : (int)std::min<double>(4096.0, std::ceil((1.5 * b.expect + 32.0) / 32.0) * 32.0);
+  const int LP = lp == 16 || lp == 32 || lp == 64 ? lp : 32;
+  int64_t np = 0;
+  for (;;) {
+    int32_t* ell = scratch_t<int32_t>(ctx, (t + "_ell").c_str(), (size_t)std::max<int64_t>(n, 1) * K);
+    if (n > 0)
+      launch(ctx, "batch_scan", [&] {
+        const unsigned g = grid_for(n * LP, 256);
+        if (LP == 64)
+          k_batch_scan<64><<<g, 256, 0, ctx->stream>>>(n, b.cfg, b.start, b.boxes, b.pos, b.species, smax, cd, cnt,
+                                                       ell, K, flag);
+        else if (LP == 16)
+          k_batch_scan<16><<<g, 256, 0, ctx->stream>>>(n, b.cfg, b.start, b.boxes, b.pos, b.species, smax, cd, cnt,
+                                                       ell, K, flag);
+        else
+          k_batch_scan<32><<<g, 256, 0, ctx->stream>>>(n, b.cfg, b.start, b.boxes, b.pos, b.species, smax, cd, cnt,
+                                                       ell, K, flag);
+      });
+    if (cap > 0) {
+      np = cap;
+      exclusive_scan(ctx, cnt, n, L.off, false, cap, flag, 1);
+      L.exact = false;
+      break;
+    }
+    np = exclusive_scan(ctx, cnt, n, L.off, true);
+    int32_t f = 0;
+    d2h(ctx, &f, flag, sizeof(int32_t));
+    sync(ctx);
+    if (!(f & 8) || K >= 4096) {
+      IPG_REQUIRE(!(f & 8), "a neighbor row exceeds 4096 entries");
+      break;
+    }
+    K *= 2;  // a row overflowed: scan again with wider rows
+    zero(ctx, flag, sizeof(int32_t));
+  }
+  L.npairs = np;
+  L.ellK = K;
+  L.ellK_req = K;
+  out.ellK = K;
+  out.npairs_a = np;
+  IPG_REQUIRE(np < ((int64_t)1 << 31), "too many SNAP pairs for 32-bit pair indices");
+  const int64_t npa = np > 0 ? np : 1;
+  L.nbr = scratch_t<int32_t>(ctx, (t + "_nbr").c_str(), npa);
+  out.center = scratch_t<int32_t>(ctx, (t + "_center").c_str(), npa);
+  out.rvec = scratch_t<double>(ctx, (t + "_rvec").c_str(), 3 * npa);
+  out.R = (need & SNAPL_R) ? scratch_t<double>(ctx, (t + "_R").c_str(), npa) : nullptr;
+  const int32_t* ell = scratch_t<int32_t>(ctx, (t + "_ell").c_str(), (size_t)std::max<int64_t>(n, 1) * K);
+  if (n > 0 && np > 0) {
+    const int FL = K >= 64 ? 64 : 32;
+    const size_t sm = sizeof(int32_t) * (size_t)(256 / FL) * K;
+    IPG_REQUIRE(sm <= 64 * 1024, "internal: neighbor row buffer too wide");
+    IpgBox none{};
+    launch(ctx, "batch_list_finish", [&] {
+      if (FL == 64)
+        k_snapc_finish<64><<<grid_for(n * 64, 256), 256, sm, ctx->stream>>>(
+            n, K, ell, cnt, L.off, b.pos, none, b.cfg, b.boxes, L.nbr, out.center, out.rvec, out.R, nullptr, flag);
+      else
+        k_snapc_finish<32><<<grid_for(n * 32, 256), 256, sm, ctx->stream>>>(
+            n, K, ell, cnt, L.off, b.pos, none, b.cfg, b.boxes, L.nbr, out.center, out.rvec, out.R, nullptr, flag);
+    });
+  }
+  if (need & (SNAPL_REVERSE | SNAPL_SYM)) {
+    out.roff = scratch_t<int64_t>(ctx, (t + "_roff").c_str(), n + 1);
+    out.rpair = scratch_t<int32_t>(ctx, (t + "_rpair").c_str(), npa);
+    build_center_and_reverse(ctx, n, np, L.off, L.nbr, nullptr, out.roff, out.rpair, t.c_str());
+  }
+  return out;
+}
+
+// Largest squared PeriodicEuclidean displacement of any atom since the
+// reference positions, as the bit pattern of a non-negative double (whose
+// unsigned order is the numeric order), so an integer max gives it exactly.
+__global__ void k_max_disp2(int64_t n, const double* __restrict__ pos, const double* __restrict__ ref, IpgBox box,
+                            uint64_t* __restrict__ out) {
+  double m = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    m = fmax(m, ipg_pe_sumsq(box, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], ref[3 * i], ref[3 * i + 1],
+                             ref[3 * i + 2]));
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax((unsigned long long*)out, (unsigned long long)__double_as_longlong(m));
+}
+
+// (Re)builds a skin list at cut + skin from the system's current positions
+// and records them as the reference.  The new buffers replace the old ones
+// only once the build succeeded.
+static void skin_build(ipg_nlist* h, const DevSystem& s) {
+  ipg_ctx* ctx = h->ctx;
+  DevList L = build_list_owned(ctx, s, h->rcut, h->kind == IPG_LIST_HALF ? KIND_HALF : KIND_FULL, false, false);
+  if (!h->ref) {
+    void* p = nullptr;
+    if (hipMalloc(&p, sizeof(double) * 3 * std::max<int64_t>(s.n, 1) + sizeof(uint64_t)) != hipSuccess) {
+      (void)hipGetLastError();
+      (void)hipFree(L.off);
+      (void)hipFree(L.nbr);
+      fail(IPG_ERR_NOMEM, "hipMalloc failed for a skin list");
+    }
+    h->ref = (double*)p;
+    h->dmax = (uint64_t*)(h->ref + 3 * std::max<int64_t>(s.n, 1));
+  }
+  if (s.n > 0)
+    IPG_HIP(hipMemcpyAsync(h->ref, s.pos, sizeof(double) * 3 * s.n, hipMemcpyDeviceToDevice, ctx->stream));
+  IPG_HIP(hipStreamSynchronize(ctx->stream));  // the old list may still be read by enqueued work
+  if (h->list.off) (void)hipFree(h->list.off);
+  if (h->list.nbr) (void)hipFree(h->list.nbr);
+  if (h->list.disp) (void)hipFree(h->list.disp);
+  h->list = L;
+  h->box = s.box;
+  h->builds++;
+}
+
 }  // namespace ipg
 
 using namespace ipg;
@@ -934,8 +1113,71 @@ int ipg_nlist_build_dev(ipg_ctx* ctx, const ipg_system* sys, double rcut, int ki
     }
     h->ctx = ctx;
     h->rcut = rcut;
+    h->cut = rcut;
     h->kind = kind;
+    h->box = s.box;
+    h->builds = 1;
     *out = h;
+  });
+}
+
+int ipg_nlist_build_skin_dev(ipg_ctx* ctx, const ipg_system* sys, double rcut, double skin, int kind,
+                             ipg_nlist** out) {
+  return guard([&] {
+    IPG_REQUIRE(ctx && sys && out, "NULL argument");
+    IPG_REQUIRE(kind == IPG_LIST_HALF || kind == IPG_LIST_FULL, "kind must be IPG_LIST_HALF or IPG_LIST_FULL");
+    IPG_REQUIRE(std::isfinite(rcut) && rcut > 0, "rcutoff must be finite and > 0");
+    IPG_REQUIRE(std::isfinite(skin) && skin > 0, "skin must be finite and > 0");
+    IPG_HIP(hipSetDevice(ctx->device));
+    DevSystem s = make_dev_system(sys);
+    auto* h = new ipg_nlist();
+    h->ctx = ctx;
+    h->cut = rcut;
+    h->skin = skin;
+    h->rcut = rcut + skin;
+    h->kind = kind;
+    try {
+      skin_build(h, s);
+    } catch (...) {
+      if (h->ref) (void)hipFree(h->ref);
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int ipg_nlist_update_dev(ipg_nlist* list, const ipg_system* sys, int32_t* rebuilt) {
+  return guard([&] {
+    IPG_REQUIRE(list && sys, "NULL argument");
+    IPG_REQUIRE(list->skin > 0, "not a skin list (build it with ipg_nlist_build_skin_dev)");
+    ipg_ctx* ctx = list->ctx;
+    IPG_HIP(hipSetDevice(ctx->device));
+    DevSystem s = make_dev_system(sys);
+    IPG_REQUIRE(s.n == list->list.n, "natoms differs from the list's");
+    bool same_box = true;
+    for (int d = 0; d < 3; d++)
+      same_box = same_box && s.box.per[d] == list->box.per[d] && (!s.box.per[d] || s.box.L[d] == list->box.L[d]);
+    bool rebuild = !same_box;
+    if (!rebuild && s.n > 0) {
+      // every pair within cut now was within cut + 2 max|dx| at the last
+      // build; the margin covers the rounding of the distance sums
+      const double half = 0.5 * list->skin - 1e-9 * list->rcut;
+      zero(ctx, list->dmax, sizeof(uint64_t));
+      launch(ctx, "nlist_max_disp", [&] {
+        k_max_disp2<<<std::min<unsigned>(grid_for(s.n, 256), 1024), 256, 0, ctx->stream>>>(s.n, s.pos, list->ref,
+                                                                                           s.box, list->dmax);
+      });
+      uint64_t bits = 0;
+      d2h(ctx, &bits, list->dmax, sizeof(uint64_t));
+      sync(ctx);
+      double m2;
+      std::memcpy(&m2, &bits, sizeof(double));
+      rebuild = !(m2 <= half * half) || half <= 0.0;
+    }
+    if (rebuild) skin_build(list, s);
+    list->updates++;
+    if (rebuilt) *rebuilt = rebuild ? 1 : 0;
   });
 }
 
@@ -957,6 +1199,7 @@ int ipg_nlist_destroy(ipg_nlist* list) {
     if (list->list.off) (void)hipFree(list->list.off);
     if (list->list.nbr) (void)hipFree(list->list.nbr);
     if (list->list.disp) (void)hipFree(list->list.disp);
+    if (list->ref) (void)hipFree(list->ref);
     delete list;
   });
 }

@@ -356,7 +356,7 @@ int ipg_pair_eval_dev(ipg_ctx* ctx, const ipg_nlist* full_list, const ipg_system
   return guard([&] {
     IPG_REQUIRE(ctx && sys && full_list, "NULL argument");
     IPG_REQUIRE(full_list->kind == IPG_LIST_FULL, "ipg_pair_eval_dev needs an IPG_LIST_FULL list");
-    IPG_REQUIRE(max_rcut(terms, nterms) <= full_list->rcut, "list cutoff is below a term's rcut");
+    IPG_REQUIRE(max_rcut(terms, nterms) <= full_list->cut, "list cutoff is below a term's rcut");
     IPG_HIP(hipSetDevice(ctx->device));
     DevSystem s = make_dev_system(sys);
     pair_dev(ctx, s, terms, nterms, precision, energy, forces, virial, &full_list->list, full_list->rcut);

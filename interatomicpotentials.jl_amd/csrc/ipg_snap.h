@@ -1,6 +1,7 @@
 // ipg_snap.h -- SNAP tables, device views and the forward-mode Dual scalar.
 #pragma once
 #include <array>
+#include <functional>
 #include <type_traits>
 
 #include "ipg_nlist.h"
@@ -91,6 +92,8 @@ struct SnapRun {
   const int32_t* rev = nullptr;   // reverse pair of each pair (symmetric CUTOFF list) or null
   const int64_t* d_np = nullptr;  // device pair count when npairs is a capacity (else null)
   double avg_nbr = -1.0;          // mean neighbors per atom for launch heuristics (<0: npairs/n)
+  int apw = 0;                    // U kernel atoms per wave (1 or 2; 0: from avg_nbr)
+  int dbdr_threads = 0;           // dB kernel block size (64 or 128; 0: 128)
 };
 
 // ---------------------------------------------------------------------------
@@ -172,42 +175,8 @@ void snap_launch_dbdr(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const S*
 // [ceil(n/32)][idxu_max][32][2]; beta expanded [nspecies][ncoeff].
 void snap_launch_zy32(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const float* U, const double* beta, float* Y,
                       double* eatom);
-
-// A captured energy+force step (lists in capacity mode, no host
-// synchronization inside) replayed while its inputs stay the same buffers.
-struct SnapGraph {
-  hipGraphExec_t exec = nullptr;
-  // what the captured work depends on
-  int64_t n = -1;
-  const double* pos = nullptr;
-  const int32_t* species = nullptr;
-  const int32_t* appear = nullptr;
-  IpgBox box{};
-  int mode = -1, path = -1;
-  const double* beta = nullptr;
-  double beta0 = 0.0;
-  double* energy = nullptr;
-  double* forces = nullptr;
-  bool prof = false;
-  uint64_t gen = 0;  // scratch generation the captured pointers belong to
-  int64_t capA = 0, capB = 0;
-  int ellK = 0;  // neighbor row width of the CUTOFF list's first pass
-  double avg_nbr = -1.0;
-  int32_t* flag = nullptr;  // bits of the last replay: 1, 2, 8 capacity; 4 missing reverse pair
-  std::vector<ipg_ctx::Pending> events;  // profiling event pairs inside the graph
-  bool disabled[2] = {false, false};     // capture failed (without / with profiling events)
-  std::string reason;
-  int64_t captures = 0, replays = 0, overflows = 0;
-};
+// FP32 descriptors from the atom-blocked U: Z [N][idxz_max][2] (FP32) and
+// B [N][ncoeff] (FP64 sums).
+void snap_launch_zb32(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const float* U, float* Z, double* B);
 
 }  // namespace ipg
-
-struct ipg_snap {
-  ipg_ctx* ctx = nullptr;
-  ipg::SnapHost host;
-  ipg::SnapDev dev{};
-  void* dmem = nullptr;  // tables
-  double* beta_dev = nullptr;  // expanded beta (ncoeff * nspecies), owned
-  int64_t beta_cap = 0;
-  ipg::SnapGraph graph;
-};

@@ -4,10 +4,11 @@
 // the same j1 >= j2, j = |j1-j2|..min(J, j1+j2) step 2 order; the
 // Clebsch-Gordan coefficients from the Racah closed form); the derived
 // tables (Y coefficients, dB half-sum terms) are this engine's own.
-#include "ipg_snap.h"
+#include "ipg_snap_driver.h"
 #include "ipg_snap_kernels.h"
 
 #include <algorithm>
+#include <functional>
 
 namespace ipg {
 
@@ -338,7 +339,8 @@ static void force_gather(ipg_ctx* ctx, const SnapRun& R, const double* dedr, dou
 // snap.jl:264-265): block s(a) gets the atom's own pairs, block s gets minus
 // the reverse pairs whose center has species s.  One pass over the own row
 // and one over the reverse row, each in ascending pair order.
-__global__ void k_db_gather(SnapDev P, SnapRun R, const double* __restrict__ dblist, double* __restrict__ dB) {
+template <typename T>
+__global__ void k_db_gather(SnapDev P, SnapRun R, const T* __restrict__ dblist, double* __restrict__ dB) {
   const int64_t a = blockIdx.x;
   const int M = P.ncoeff * P.nspecies;
   const int sa = R.species ? R.species[a] : 0;
@@ -348,14 +350,14 @@ __global__ void k_db_gather(SnapDev P, SnapRun R, const double* __restrict__ dbl
 #pragma unroll
     for (int s = 0; s < 8; s++) acc[s] = 0.0;
     double own = 0.0;
-    for (int64_t q = q0; q < q1; q++) own += dblist[q * 3 * P.ncoeff + t];
+    for (int64_t q = q0; q < q1; q++) own += (double)dblist[q * 3 * P.ncoeff + t];
 #pragma unroll
     for (int s = 0; s < 8; s++)
       if (s == sa) acc[s] = own;
     for (int64_t r = r0; r < r1; r++) {
       const int64_t q = R.rpair[r];
       const int sc = R.species ? R.species[R.center[q]] : 0;
-      const double v = dblist[q * 3 * P.ncoeff + t];
+      const double v = (double)dblist[q * 3 * P.ncoeff + t];
 #pragma unroll
       for (int s = 0; s < 8; s++)
         if (s == sc) acc[s] -= v;
@@ -419,7 +421,7 @@ struct SnapCall {
   int32_t* flag = nullptr;  // device bits: 1, 2, 8 capacity overflow; 4 missing reverse pair
 };
 
-static SnapCut make_cut(const SnapHost& h, int mode) {
+SnapCut snap_make_cut(const SnapHost& h, int mode) {
   SnapCut c{};
   c.ns = h.nspecies;
   double cutmax = 0.0;
@@ -445,7 +447,7 @@ static SnapCall snap_prepare(ipg_snap* sn, const DevSystem& s, int mode, const i
   SnapCall c;
   c.flag = scratch_t<int32_t>(sn->ctx, "snap_flag", 1);
   zero(sn->ctx, c.flag, sizeof(int32_t));
-  SnapCut cut = make_cut(sn->host, mode);
+  SnapCut cut = snap_make_cut(sn->host, mode);
   const int32_t* ap = d_appear ? d_appear : s.species;
   if (cut.reference && !ap) {
     // all atoms are species 0: element index 0 everywhere
@@ -525,10 +527,16 @@ static double* run_dblist(ipg_snap* sn, const SnapCall& c, const double* Z) {
   return dbl;
 }
 
+template <typename T>
+void snap_gather_db(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const T* dbl, double* dB) {
+  if (R.n == 0) return;
+  launch(ctx, "snap_db_gather", [&] { k_db_gather<T><<<(unsigned)R.n, 64, 0, ctx->stream>>>(P, R, dbl, dB); });
+}
+template void snap_gather_db<float>(ipg_ctx*, const SnapDev&, const SnapRun&, const float*, double*);
+template void snap_gather_db<double>(ipg_ctx*, const SnapDev&, const SnapRun&, const double*, double*);
+
 static void gather_db(ipg_snap* sn, const SnapCall& c, const double* dbl, double* dB) {
-  if (c.R.n == 0) return;
-  launch(sn->ctx, "snap_db_gather",
-         [&] { k_db_gather<<<(unsigned)c.R.n, 64, 0, sn->ctx->stream>>>(sn->dev, c.R, dbl, dB); });
+  snap_gather_db<double>(sn->ctx, sn->dev, c.R, dbl, dB);
 }
 
 // Energy and forces on the device; d_energy/d_forces are device pointers.
@@ -600,7 +608,7 @@ static void fix_asymmetric(ipg_snap* sn, SnapCall& c, double* d_forces) {
 // ---------------------------------------------------------------------------
 // captured step: energy + forces of one system, replayed from a HIP graph
 // ---------------------------------------------------------------------------
-static void graph_release(ipg_ctx* ctx, SnapGraph& g) {
+void graph_release(ipg_ctx* ctx, SnapGraph& g) {
   if (g.exec) {
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipGraphExecDestroy(g.exec);
@@ -620,7 +628,67 @@ static bool graph_matches(const SnapGraph& g, const ipg_ctx* ctx, const DevSyste
   for (int k = 0; k < 3; k++)
     if (g.box.per[k] != s.box.per[k] || g.box.L[k] != s.box.L[k]) return false;
   return g.mode == mode && g.path == path && g.beta == d_beta && g.beta0 == beta0 && g.energy == d_energy &&
-         g.forces == d_forces && g.prof == ctx->prof && g.gen == ctx->scratch_gen;
+         g.forces == d_forces && g.prof == ctx->prof && g.prof_gen == ctx->prof_gen && g.gen == ctx->scratch_gen;
+}
+
+// Allocation pass (launches skipped) at the step's sizes, then capture of
+// the step into g.exec.  On failure the graph stays off for this profiling
+// state, with the reason kept.  Sets g.events and g.gen on success.
+bool graph_capture(ipg_ctx* ctx, SnapGraph& g, const std::function<void()>& step) {
+  ctx->dry = true;
+  try {
+    step();
+  } catch (...) {
+    ctx->dry = false;
+    throw;
+  }
+  ctx->dry = false;
+  const uint64_t gen = ctx->scratch_gen;
+  if (ctx->prof)  // events for the graph's nodes are created before the capture starts
+    while (ctx->free_events.size() < 256) {
+      hipEvent_t e = nullptr;
+      IPG_HIP(hipEventCreate(&e));
+      ctx->free_events.push_back(e);
+    }
+  std::vector<ipg_ctx::Pending> ev;
+  std::string why;
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    ctx->capture_events = &ev;
+    try {
+      step();
+    } catch (const std::exception& x) {
+      why = x.what();
+    }
+    ctx->capture_events = nullptr;
+    hipError_t e2 = hipStreamEndCapture(ctx->stream, &graph);
+    if (why.empty() && e2 != hipSuccess) why = std::string("hipStreamEndCapture: ") + hipGetErrorString(e2);
+    if (why.empty() && ctx->scratch_gen != gen) why = "scratch reallocated during capture";
+    if (why.empty()) {
+      e2 = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+      if (e2 != hipSuccess) {
+        g.exec = nullptr;
+        why = std::string("hipGraphInstantiate: ") + hipGetErrorString(e2);
+      }
+    }
+    if (graph) (void)hipGraphDestroy(graph);
+  } else {
+    why = std::string("hipStreamBeginCapture: ") + hipGetErrorString(e);
+  }
+  if (!why.empty()) {
+    (void)hipGetLastError();
+    for (auto& p : ev) {
+      ctx->free_events.push_back(p.a);
+      ctx->free_events.push_back(p.b);
+    }
+    g.disabled[ctx->prof ? 1 : 0] = true;
+    g.reason = why;
+    return false;
+  }
+  g.events = std::move(ev);
+  g.gen = gen;
+  return true;
 }
 
 // Energy and forces into device buffers.  For a periodic system with graphs
@@ -671,59 +739,7 @@ static void snap_energy_forces_call(ipg_snap* sn, const DevSystem& s, int mode, 
     cc.R.avg_nbr = g.avg_nbr;
     energy_forces_dev(sn, cc, d_beta, beta0, path, d_energy, d_forces);
   };
-  // allocation pass at the capacities (nothing is launched)
-  ctx->dry = true;
-  try {
-    step();
-  } catch (...) {
-    ctx->dry = false;
-    throw;
-  }
-  ctx->dry = false;
-  const uint64_t gen = ctx->scratch_gen;
-  if (ctx->prof)  // events for the graph's nodes are created before the capture starts
-    while (ctx->free_events.size() < 256) {
-      hipEvent_t e = nullptr;
-      IPG_HIP(hipEventCreate(&e));
-      ctx->free_events.push_back(e);
-    }
-  std::vector<ipg_ctx::Pending> ev;
-  std::string why;
-  hipGraph_t graph = nullptr;
-  hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
-  if (e == hipSuccess) {
-    ctx->capture_events = &ev;
-    try {
-      step();
-    } catch (const std::exception& x) {
-      why = x.what();
-    }
-    ctx->capture_events = nullptr;
-    hipError_t e2 = hipStreamEndCapture(ctx->stream, &graph);
-    if (why.empty() && e2 != hipSuccess) why = std::string("hipStreamEndCapture: ") + hipGetErrorString(e2);
-    if (why.empty() && ctx->scratch_gen != gen) why = "scratch reallocated during capture";
-    if (why.empty()) {
-      e2 = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
-      if (e2 != hipSuccess) {
-        g.exec = nullptr;
-        why = std::string("hipGraphInstantiate: ") + hipGetErrorString(e2);
-      }
-    }
-    if (graph) (void)hipGraphDestroy(graph);
-  } else {
-    why = std::string("hipStreamBeginCapture: ") + hipGetErrorString(e);
-  }
-  if (!why.empty()) {
-    (void)hipGetLastError();
-    for (auto& p : ev) {
-      ctx->free_events.push_back(p.a);
-      ctx->free_events.push_back(p.b);
-    }
-    g.disabled[ctx->prof ? 1 : 0] = true;
-    g.reason = why;
-    return;  // this call's result came from the exact run
-  }
-  g.events = std::move(ev);
+  if (!graph_capture(ctx, g, step)) return;  // this call's result came from the exact run
   g.n = s.n;
   g.pos = s.pos;
   g.species = s.species;
@@ -736,7 +752,7 @@ static void snap_energy_forces_call(ipg_snap* sn, const DevSystem& s, int mode, 
   g.energy = d_energy;
   g.forces = d_forces;
   g.prof = ctx->prof;
-  g.gen = gen;
+  g.prof_gen = ctx->prof_gen;
   g.captures++;
 }
 

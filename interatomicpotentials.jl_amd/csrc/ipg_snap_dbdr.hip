@@ -98,11 +98,11 @@ __global__ __launch_bounds__(DBDR_THREADS, 1) void k_snap_dbdr(SnapDev P, SnapRu
   __shared__ S Zs[2 * NZ];
   const int64_t a = blockIdx.x;
   const S* zg = Zg + a * (int64_t)NZ * 2;
-  for (int e = threadIdx.x; e < 2 * NZ; e += DBDR_THREADS) Zs[e] = zg[e];
+  for (int e = threadIdx.x; e < 2 * NZ; e += blockDim.x) Zs[e] = zg[e];
   __syncthreads();
   const int64_t q0 = R.off[a], q1 = R.off[a + 1];
   const int si = R.species ? R.species[a] : 0;
-  for (int64_t it = threadIdx.x; it < 3 * (q1 - q0); it += DBDR_THREADS) {
+  for (int64_t it = threadIdx.x; it < 3 * (q1 - q0); it += blockDim.x) {
     const int64_t q = q0 + it / 3;
     const int k = (int)(it % 3);
     const int32_t jn = R.nbr[q];
@@ -145,7 +145,8 @@ __global__ __launch_bounds__(DBDR_THREADS, 1) void k_snap_dbdr(SnapDev P, SnapRu
 
 template <int J, typename S>
 static void launch_dbdr_j(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const S* Z, S* dblist) {
-  k_snap_dbdr<J, S><<<(unsigned)R.n, DBDR_THREADS, 0, ctx->stream>>>(P, R, Z, dblist);
+  const int threads = R.dbdr_threads == 64 ? 64 : DBDR_THREADS;  // lanes loop over the (pair, component) items
+  k_snap_dbdr<J, S><<<(unsigned)R.n, threads, 0, ctx->stream>>>(P, R, Z, dblist);
 }
 
 template <typename S>

@@ -765,7 +765,8 @@ template <typename S>
 void snap_launch_ui(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, S* ulisttot) {
   if (R.n == 0) return;
   const double avg = R.avg_nbr >= 0.0 ? R.avg_nbr : (double)R.npairs / (double)R.n;
-  const int apw = avg <= 30.0 ? 2 : 1;  // 32 lanes per atom hold up to 32 neighbors in one chunk
+  // 32 lanes per atom hold up to 32 neighbors in one chunk
+  const int apw = R.apw == 1 || R.apw == 2 ? R.apw : (avg <= 30.0 ? 2 : 1);
   const char* nm = IsDual<S>::value ? "snap_ui_dual" : (sizeof(S) == 4 ? "snap_ui_fp32" : "snap_ui_fp64");
   launch(ctx, nm, [&] {
     switch (P.J) {
@@ -829,5 +830,19 @@ void snap_launch_dedr(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const S*
   IPG_SNAP_INSTANTIATE_UD(S)                                                                                     \
   template void snap_launch_zby<S>(ipg_ctx*, const SnapDev&, const SnapRun&, const S*, const double*,            \
                                    const typename BaseOf<S>::type*, double*, double*, S*, S*, double*);
+
+// Each launcher is instantiated once, in ipg_snap_k_*.hip; every other
+// source that includes this header only calls them.
+#define IPG_SNAP_EXTERN_UD(S)                                                                                    \
+  extern template void snap_launch_ui<S>(ipg_ctx*, const SnapDev&, const SnapRun&, S*);                          \
+  extern template void snap_launch_dedr<S>(ipg_ctx*, const SnapDev&, const SnapRun&, const S*, double*, double*);
+#define IPG_SNAP_EXTERN(S)                                                                                       \
+  IPG_SNAP_EXTERN_UD(S)                                                                                          \
+  extern template void snap_launch_zby<S>(ipg_ctx*, const SnapDev&, const SnapRun&, const S*, const double*,     \
+                                          const typename BaseOf<S>::type*, double*, double*, S*, S*, double*);
+IPG_SNAP_EXTERN_UD(float)
+IPG_SNAP_EXTERN(double)
+IPG_SNAP_EXTERN(Dual<float>)
+IPG_SNAP_EXTERN(Dual<double>)
 
 }  // namespace ipg

@@ -1,337 +1,47 @@
-// ipg_snap_zy.hip -- FP32 adjoint kernel: Z (compute_z.jl:1-62), the energy
-// beta . B (compute_b.jl:67-157, linear_potential.jl:20-26) and the adjoint
-// Y = dE/dU for batches of 32 atoms, with every lane owning one atom.
-//
-// Work layout.  The output rows (j, mb), mb <= j/2, of Y are the units of
-// work; a unit sums coef(j1,j2,j) * Z_(j1,j2,j)[mb][:] over every block
-// (j1, j2) that produces level j.  Units are assigned to the waves of a
-// block at compile time (longest-processing-time first), so all control
-// flow is wave-uniform.  Inside a block (j1, j2, j) the loop over the
-// mb1 rows is the only run-time loop: the two 32-lane halves of a wave take
-// alternate mb1 rows, load row mb1 of U_j1 and row mb2 of U_j2 once into
-// registers and form every column ma of the output row from them, with the
-// Clebsch-Gordan factors of the column sums as compile-time literals.  U of
-// the batch sits in LDS as [jju][32 atoms] (conflict-free 8-byte reads).
-// Z, Y and the energy accumulate in FP64 registers; Z is never stored.
-//
-// The U and Y buffers use the atom-blocked layout [batch][jju][32][re, im]
-// written by the FP32 U kernel and read by the FP32 force kernel.
-#include "ipg_snap_kernels.h"
+// ipg_snap_zy.hip -- dispatch of the FP32 Z kernels (ipg_snap_zy.h) over
+// twojmax: the adjoint Z/energy/Y kernel and the descriptor Z/B kernel.
+#include "ipg_snap_zy.h"
 
 namespace ipg {
-namespace zy {
 
-constexpr int AB = 32;  // atoms per batch (one per lane of a half wave)
+#define IPG_ZY_EXTERN(J) IPG_ZY_LAUNCHER(extern template, J, false) IPG_ZY_LAUNCHER(extern template, J, true)
+IPG_ZY_EXTERN(0)
+IPG_ZY_EXTERN(1)
+IPG_ZY_EXTERN(2)
+IPG_ZY_EXTERN(3)
+IPG_ZY_EXTERN(4)
+IPG_ZY_EXTERN(5)
+IPG_ZY_EXTERN(6)
+IPG_ZY_EXTERN(7)
+IPG_ZY_EXTERN(8)
 
-constexpr int cmin(int a, int b) { return a < b ? a : b; }
-constexpr int cmax(int a, int b) { return a > b ? a : b; }
-constexpr double fact(int n) {
-  double r = 1.0;
-  for (int k = 2; k <= n; k++) r *= (double)k;
-  return r;
-}
-// Newton's iteration for sqrt at compile time (converges to within one ulp).
-constexpr double csqrt(double x) {
-  if (!(x > 0.0)) return 0.0;
-  double g = x < 1.0 ? 1.0 : x;
-  for (int it = 0; it < 400; it++) {
-    const double ng = 0.5 * (g + x / g);
-    if (ng == g) break;
-    g = ng;
+template <bool DESC>
+static void launch_zy(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const float* U, const double* beta, float* Y,
+                      double* eatom, float* Z, double* B) {
+  switch (P.J) {
+    case 0: launch_zy_j<0, DESC>(ctx, P, R, U, beta, Y, eatom, Z, B); break;
+    case 1: launch_zy_j<1, DESC>(ctx, P, R, U, beta, Y, eatom, Z, B); break;
+    case 2: launch_zy_j<2, DESC>(ctx, P, R, U, beta, Y, eatom, Z, B); break;
+    case 3: launch_zy_j<3, DESC>(ctx, P, R, U, beta, Y, eatom, Z, B); break;
+    case 4: launch_zy_j<4, DESC>(ctx, P, R, U, beta, Y, eatom, Z, B); break;
+    case 5: launch_zy_j<5, DESC>(ctx, P, R, U, beta, Y, eatom, Z, B); break;
+    case 6: launch_zy_j<6, DESC>(ctx, P, R, U, beta, Y, eatom, Z, B); break;
+    case 7: launch_zy_j<7, DESC>(ctx, P, R, U, beta, Y, eatom, Z, B); break;
+    default: launch_zy_j<8, DESC>(ctx, P, R, U, beta, Y, eatom, Z, B); break;
   }
-  return g;
-}
-// <j1/2 a/2, j2/2 b/2 | j/2 (a+b)/2>, Racah formula with doubled arguments
-// (the same closed form as the host table).
-constexpr double cg2(int j1, int j2, int j, int a, int b) {
-  const int c = a + b;
-  if (c < -j || c > j) return 0.0;
-  const double tri =
-      fact((j1 + j2 - j) / 2) * fact((j1 - j2 + j) / 2) * fact((-j1 + j2 + j) / 2) / fact((j1 + j2 + j) / 2 + 1);
-  const double nrm = (double)(j + 1) * fact((j1 + a) / 2) * fact((j1 - a) / 2) * fact((j2 + b) / 2) *
-                     fact((j2 - b) / 2) * fact((j + c) / 2) * fact((j - c) / 2);
-  const int zmin = cmax(0, cmax(-(j - j2 + a) / 2, -(j - j1 - b) / 2));
-  const int zmax = cmin((j1 + j2 - j) / 2, cmin((j1 - a) / 2, (j2 + b) / 2));
-  double s = 0.0;
-  for (int z = zmin; z <= zmax; z++) {
-    const double den = fact(z) * fact((j1 + j2 - j) / 2 - z) * fact((j1 - a) / 2 - z) * fact((j2 + b) / 2 - z) *
-                       fact((j - j2 + a) / 2 + z) * fact((j - j1 - b) / 2 + z);
-    s += ((z & 1) ? -1.0 : 1.0) / den;
-  }
-  return csqrt(tri) * csqrt(nrm) * s;
-}
-// table entry [m1][m2] of block (j1, j2, j)
-constexpr double cgm(int j1, int j2, int j, int m1, int m2) { return cg2(j1, j2, j, 2 * m1 - j1, 2 * m2 - j2); }
-
-constexpr bool valid(int J, int j1, int j2, int j) {
-  return j2 <= j1 && j1 <= J && j >= j1 - j2 && j <= cmin(J, j1 + j2) && ((j1 + j2 + j) % 2 == 0);
-}
-// Position of block (j1, j2, j) in the j1 >= j2, j = j1-j2 .. min(J, j1+j2)
-// step 2 order (types/prebuilt_arrays.jl:48-321): Z-block index, B index
-// (blocks with j >= j1) and Clebsch-Gordan table offset.
-struct BlockPos {
-  int blk, jjb, cgoff;
-};
-constexpr BlockPos block_pos(int J, int j1, int j2, int j) {
-  BlockPos p{0, 0, 0};
-  for (int a = 0; a <= J; a++)
-    for (int b = 0; b <= a; b++)
-      for (int c = a - b; c <= cmin(J, a + b); c += 2) {
-        if (a == j1 && b == j2 && c == j) return p;
-        p.blk++;
-        if (c >= a) p.jjb++;
-        p.cgoff += (a + 1) * (b + 1);
-      }
-  return p;
-}
-// Column-sum ranges of one output entry (the ma/mb rules of compute_z.jl).
-constexpr int m1min(int j1, int j2, int j, int m) { return cmax(0, (2 * m - j - j2 + j1) / 2); }
-constexpr int m2max(int j1, int j2, int j, int m) { return (2 * m - j - (2 * m1min(j1, j2, j, m) - j1) + j2) / 2; }
-constexpr int mcount(int j1, int j2, int j, int m) {
-  return cmin(j1, (2 * m - j + j2 + j1) / 2) - m1min(j1, j2, j, m) + 1;
-}
-
-// Cost model of a unit (j, mb): products plus row loads per mb1 step.
-constexpr int unit_cost(int J, int j, int mb) {
-  int cost = 0;
-  for (int j1 = 0; j1 <= J; j1++)
-    for (int j2 = 0; j2 <= j1; j2++) {
-      if (!valid(J, j1, j2, j)) continue;
-      int pr = 0;
-      for (int ma = 0; ma <= j; ma++) pr += mcount(j1, j2, j, ma);
-      const int nb = mcount(j1, j2, j, mb);
-      cost += nb * (pr + (j1 + j2 + 2) / 2) + 4 * (j + 1);
-    }
-  return cost;
-}
-constexpr int num_units(int J) {
-  int u = 0;
-  for (int j = 0; j <= J; j++) u += j / 2 + 1;
-  return u;
-}
-
-template <int J, int NW>
-struct Plan {
-  static constexpr int NU = num_units(J);
-  int uj[NU] = {}, umb[NU] = {}, owner[NU] = {};
-  constexpr Plan() {
-    int cost[NU] = {}, order[NU] = {};
-    int u = 0;
-    for (int j = 0; j <= J; j++)
-      for (int mb = 0; 2 * mb <= j; mb++) {
-        uj[u] = j;
-        umb[u] = mb;
-        cost[u] = unit_cost(J, j, mb);
-        order[u] = u;
-        u++;
-      }
-    for (int a = 0; a < NU; a++)  // by decreasing cost
-      for (int b = a + 1; b < NU; b++)
-        if (cost[order[b]] > cost[order[a]]) {
-          const int t = order[a];
-          order[a] = order[b];
-          order[b] = t;
-        }
-    int load[NW] = {};
-    for (int k = 0; k < NU; k++) {
-      int best = 0;
-      for (int w = 1; w < NW; w++)
-        if (load[w] < load[best]) best = w;
-      owner[order[k]] = best;
-      load[best] += cost[order[k]];
-    }
-  }
-};
-
-// One block (j1, j2, j) of output row mb: Z row (this half's mb1 rows),
-// its energy share and its Y contribution.
-template <int J, int j1, int j2, int j, int mb>
-__device__ __forceinline__ void triple(const float2* __restrict__ UL, const float* __restrict__ cgL, int h, int at,
-                                       const double* __restrict__ ycs, const double* __restrict__ bts,
-                                       const SnapDev& P, double* yr, double* yi, double& e) {
-  constexpr int mb1lo = m1min(j1, j2, j, mb);
-  constexpr int mb2hi = m2max(j1, j2, j, mb);
-  constexpr int nb = mcount(j1, j2, j, mb);
-  constexpr BlockPos bp = block_pos(J, j1, j2, j);
-  double zr[j + 1], zi[j + 1];
-  static_for<0, j + 1>([&](auto mc) {
-    zr[decltype(mc)::value] = 0.0;
-    zi[decltype(mc)::value] = 0.0;
-  });
-  for (int ib = h; ib < nb; ib += 2) {
-    const int mb1 = mb1lo + ib, mb2 = mb2hi - ib;
-    const float2* r1 = UL + (ub_of(j1) + mb1 * (j1 + 1)) * AB + at;
-    const float2* r2 = UL + (ub_of(j2) + mb2 * (j2 + 1)) * AB + at;
-    float2 u1[j1 + 1], u2[j2 + 1];
-    static_for<0, j1 + 1>([&](auto k) { u1[decltype(k)::value] = r1[decltype(k)::value * AB]; });
-    static_for<0, j2 + 1>([&](auto k) { u2[decltype(k)::value] = r2[decltype(k)::value * AB]; });
-    const double cb = (double)cgL[bp.cgoff + mb1 * (j2 + 1) + mb2];
-    static_for<0, j + 1>([&](auto mc) {
-      constexpr int ma = decltype(mc)::value;
-      constexpr int lo = m1min(j1, j2, j, ma), hi2 = m2max(j1, j2, j, ma), na = mcount(j1, j2, j, ma);
-      float sr = 0.f, si = 0.f;
-      static_for<0, na>([&](auto ic) {
-        constexpr int ma1 = lo + decltype(ic)::value, ma2 = hi2 - decltype(ic)::value;
-        constexpr float c = (float)cgm(j1, j2, j, ma1, ma2);
-        const float2 p = u1[ma1], q = u2[ma2];
-        sr += c * (p.x * q.x - p.y * q.y);
-        si += c * (p.x * q.y + p.y * q.x);
-      });
-      zr[ma] += cb * (double)sr;
-      zi[ma] += cb * (double)si;
-    });
-  }
-  if constexpr (j >= j1) {
-    // this half's share of B_(j1,j2,j) = 2 sum_half Re(conj(U_j) Z) (the
-    // middle row of an even level counts its first half and half its centre)
-    const float2* uj = UL + (ub_of(j) + mb * (j + 1)) * AB + at;
-    double bs = 0.0;
-    static_for<0, j + 1>([&](auto mc) {
-      constexpr int ma = decltype(mc)::value;
-      constexpr double wgt = (2 * mb < j) ? 1.0 : (ma < mb ? 1.0 : (ma == mb ? 0.5 : 0.0));
-      if constexpr (wgt != 0.0) {
-        const float2 uu = uj[ma * AB];
-        bs += wgt * ((double)uu.x * zr[ma] + (double)uu.y * zi[ma]);
-      }
-    });
-    double ev = 2.0 * bs;
-    if constexpr (mb == 0)
-      if (h == 0 && P.bzero) ev -= P.bzero_j[j];
-    e += bts[bp.jjb] * ev;
-  }
-  const double cf = ycs[bp.blk];
-  static_for<0, j + 1>([&](auto mc) {
-    yr[decltype(mc)::value] += cf * zr[decltype(mc)::value];
-    yi[decltype(mc)::value] += cf * zi[decltype(mc)::value];
-    // pin the update here: otherwise it is sunk to the unit's end and the
-    // Z rows of every block stay live until then
-    asm volatile("" : "+v"(yr[decltype(mc)::value]), "+v"(yi[decltype(mc)::value]));
-  });
-  asm volatile("" : "+v"(e));
-}
-
-template <int J, int j, int mb>
-__device__ __forceinline__ void unit(const float2* __restrict__ UL, const float* __restrict__ cgL, int h, int at,
-                                     const double* __restrict__ ycs, const double* __restrict__ bts,
-                                     const SnapDev& P, double& e, float2* __restrict__ Yrow, bool store) {
-  double yr[j + 1], yi[j + 1];
-  static_for<0, j + 1>([&](auto mc) {
-    yr[decltype(mc)::value] = 0.0;
-    yi[decltype(mc)::value] = 0.0;
-  });
-  static_for<0, J + 1>([&](auto c1) {
-    constexpr int j1 = decltype(c1)::value;
-    static_for<0, j1 + 1>([&](auto c2) {
-      constexpr int j2 = decltype(c2)::value;
-      if constexpr (valid(J, j1, j2, j)) triple<J, j1, j2, j, mb>(UL, cgL, h, at, ycs, bts, P, yr, yi, e);
-    });
-  });
-  // the two halves hold partial sums over alternate mb1 rows
-  static_for<0, j + 1>([&](auto mc) {
-    constexpr int ma = decltype(mc)::value;
-    yr[ma] += __shfl_xor(yr[ma], 32, 64);
-    yi[ma] += __shfl_xor(yi[ma], 32, 64);
-  });
-  if (!store) return;
-  // half h stores the columns ma = 2k + h (one instruction per column pair)
-  static_for<0, (j + 2) / 2>([&](auto kc) {
-    constexpr int m0 = 2 * decltype(kc)::value;
-    if constexpr (m0 + 1 <= j) {
-      const float2 v = h ? make_float2((float)yr[m0 + 1], (float)yi[m0 + 1]) : make_float2((float)yr[m0], (float)yi[m0]);
-      Yrow[(m0 + h) * AB] = v;
-    } else {
-      if (h == 0) Yrow[m0 * AB] = make_float2((float)yr[m0], (float)yi[m0]);
-    }
-  });
-}
-
-}  // namespace zy
-
-template <int J, int NW>
-__global__ __launch_bounds__(64 * NW) void k_snap_zy32(SnapDev P, int64_t n, const int32_t* __restrict__ species,
-                                                       const float2* __restrict__ Ub,
-                                                       const double* __restrict__ beta, float2* __restrict__ Yb,
-                                                       double* __restrict__ eatom) {
-  constexpr int IDXU = ub_of(J + 1);
-  constexpr zy::Plan<J, NW> plan{};
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  float2* UL = reinterpret_cast<float2*>(smem_raw);
-  double* esum = reinterpret_cast<double*>(UL + IDXU * zy::AB);
-  double* ycL = esum + NW * zy::AB;                 // Y coefficients [nspecies][nblk]
-  double* btL = ycL + P.nspecies * P.nblk;          // beta [nspecies][ncoeff]
-  float* cgL = reinterpret_cast<float*>(btL + P.nspecies * P.ncoeff);
-  const int tid = threadIdx.x;
-  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int h = (tid >> 5) & 1, at = tid & 31;
-  const int64_t batch = blockIdx.x;
-  const int64_t a = batch * zy::AB + at;
-  const bool valid = a < n;
-  const float2* ug = Ub + batch * (int64_t)IDXU * zy::AB;
-  for (int e = tid; e < IDXU * zy::AB; e += 64 * NW) UL[e] = ug[e];
-  for (int e = tid; e < P.idxcg_max; e += 64 * NW) cgL[e] = P.cg_f[e];
-  for (int e = tid; e < P.nspecies * P.ncoeff; e += 64 * NW) btL[e] = beta[e];
-  // Y coefficient of each block and species: the beta entries whose dB/dr
-  // half-sums use the block, with their multiplicities
-  for (int e = tid; e < P.nspecies * P.nblk; e += 64 * NW) {
-    const int sp = e / P.nblk;
-    const YBlk* yb = P.yblk + e % P.nblk;
-    double c = 0.0;
-    for (int k = 0; k < yb->n; k++) c += beta[sp * P.ncoeff + yb->jjb[k]] * yb->fac[k];
-    ycL[e] = c;
-  }
-  __syncthreads();
-  const int s = valid && species ? species[a] : 0;
-  const double* ycs = ycL + s * P.nblk;
-  const double* bts = btL + s * P.ncoeff;
-  float2* yb = Yb + batch * (int64_t)IDXU * zy::AB + at;
-  double e = 0.0;
-  static_for<0, zy::Plan<J, NW>::NU>([&](auto uc) {
-    constexpr int u = decltype(uc)::value;
-    constexpr int j = plan.uj[u], mb = plan.umb[u];
-    if (w == plan.owner[u])
-      zy::unit<J, j, mb>(UL, cgL, h, at, ycs, bts, P, e, yb + (ub_of(j) + mb * (j + 1)) * zy::AB, valid);
-  });
-  e += __shfl_xor(e, 32, 64);
-  if (h == 0) esum[w * zy::AB + at] = e;
-  __syncthreads();
-  if (w == 0 && h == 0 && valid) {
-    double t = 0.0;
-    for (int k = 0; k < NW; k++) t += esum[k * zy::AB + at];
-    eatom[a] = t;
-  }
-}
-
-template <int J>
-static void launch_zy_j(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const float* U, const double* beta,
-                        float* Y, double* eatom) {
-  constexpr int NU = zy::num_units(J);
-  constexpr int NW = NU < 12 ? NU : 12;
-  const size_t sm = sizeof(float2) * (size_t)ub_of(J + 1) * zy::AB + sizeof(double) * NW * zy::AB +
-                    sizeof(double) * (size_t)P.nspecies * (P.nblk + P.ncoeff) + sizeof(float) * (size_t)P.idxcg_max;
-  const unsigned grid = (unsigned)((R.n + zy::AB - 1) / zy::AB);
-  if (sm > 64 * 1024)
-    IPG_HIP(hipFuncSetAttribute((const void*)k_snap_zy32<J, NW>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
-  k_snap_zy32<J, NW><<<grid, 64 * NW, sm, ctx->stream>>>(P, R.n, R.species, reinterpret_cast<const float2*>(U), beta,
-                                                         reinterpret_cast<float2*>(Y), eatom);
 }
 
 void snap_launch_zy32(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const float* U, const double* beta, float* Y,
                       double* eatom) {
   if (R.n == 0) return;
-  IPG_REQUIRE(!P.bnorm, "internal: bnorm is not supported by the FP32 adjoint kernel");
-  launch(ctx, "snap_zy_fp32", [&] {
-    switch (P.J) {
-      case 0: launch_zy_j<0>(ctx, P, R, U, beta, Y, eatom); break;
-      case 1: launch_zy_j<1>(ctx, P, R, U, beta, Y, eatom); break;
-      case 2: launch_zy_j<2>(ctx, P, R, U, beta, Y, eatom); break;
-      case 3: launch_zy_j<3>(ctx, P, R, U, beta, Y, eatom); break;
-      case 4: launch_zy_j<4>(ctx, P, R, U, beta, Y, eatom); break;
-      case 5: launch_zy_j<5>(ctx, P, R, U, beta, Y, eatom); break;
-      case 6: launch_zy_j<6>(ctx, P, R, U, beta, Y, eatom); break;
-      case 7: launch_zy_j<7>(ctx, P, R, U, beta, Y, eatom); break;
-      default: launch_zy_j<8>(ctx, P, R, U, beta, Y, eatom); break;
-    }
-  });
+  IPG_REQUIRE(!P.bnorm, "internal: bnorm is not supported by the FP32 Z kernels");
+  launch(ctx, "snap_zy_fp32", [&] { launch_zy<false>(ctx, P, R, U, beta, Y, eatom, nullptr, nullptr); });
+}
+
+void snap_launch_zb32(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const float* U, float* Z, double* B) {
+  if (R.n == 0) return;
+  IPG_REQUIRE(!P.bnorm, "internal: bnorm is not supported by the FP32 Z kernels");
+  launch(ctx, "snap_zb_fp32", [&] { launch_zy<true>(ctx, P, R, U, nullptr, nullptr, nullptr, Z, B); });
 }
 
 }  // namespace ipg

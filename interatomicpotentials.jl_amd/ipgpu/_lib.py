@@ -32,7 +32,8 @@ EXPORTED = [
     "ipg_snap_create", "ipg_snap_destroy", "ipg_snap_info", "ipg_snap_neighborlist", "ipg_snap_local_descriptors",
     "ipg_snap_force_descriptors", "ipg_snap_all_descriptors", "ipg_snap_energy_forces",
     "ipg_snap_energy_forces_dev", "ipg_snap_force_vjp", "ipg_ctx_set_graphs", "ipg_snap_graph_info",
-    "ipg_device_copy",
+    "ipg_device_copy", "ipg_snap_batch_descriptors", "ipg_snap_batch_descriptors_dev", "ipg_snap_batch_info",
+    "ipg_nlist_build_skin_dev", "ipg_nlist_update_dev", "ipg_prof_select",
 ]
 
 
@@ -107,6 +108,7 @@ def load():
             "ipg_memcpy_d2h": (C.c_int, [P, P, P, C.c_size_t]),
             "ipg_prof_enable": (C.c_int, [P, C.c_int]),
             "ipg_prof_reset": (C.c_int, [P]),
+            "ipg_prof_select": (C.c_int, [P, C.c_char_p]),
             "ipg_prof_query": (C.c_int, [P, C.c_char_p, C.POINTER(i64), C.POINTER(d)]),
             "ipg_neighborlist": (C.c_int, [P, pS, d, C.c_int, P, P, P, i64, C.POINTER(i64)]),
             "ipg_nlist_build_dev": (C.c_int, [P, pS, d, C.c_int, C.POINTER(P)]),
@@ -129,6 +131,12 @@ def load():
             "ipg_device_copy": (C.c_int, [P, P, P, C.c_size_t]),
             "ipg_snap_graph_info": (C.c_int, [P, C.POINTER(i32), C.POINTER(i64), C.POINTER(i64), C.POINTER(i64),
                                               C.c_char_p, i64]),
+            "ipg_snap_batch_descriptors": (C.c_int, [P, i64, pS, C.c_int, P, P, P, P]),
+            "ipg_snap_batch_descriptors_dev": (C.c_int, [P, i64, P, P, P, P, P, C.c_int, P, P, P, P]),
+            "ipg_nlist_build_skin_dev": (C.c_int, [P, pS, d, d, C.c_int, C.POINTER(P)]),
+            "ipg_nlist_update_dev": (C.c_int, [P, pS, C.POINTER(i32)]),
+            "ipg_snap_batch_info": (C.c_int, [P, C.c_int, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
+                                              C.POINTER(i32), C.POINTER(i64)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

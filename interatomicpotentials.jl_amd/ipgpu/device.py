@@ -80,3 +80,9 @@ def prof_query(name):
     ms = C.c_double()
     check(_lib.load().ipg_prof_query(_lib.default_context(), name.encode(), C.byref(n), C.byref(ms)))
     return n.value, ms.value
+
+
+def prof_select(names=None):
+    """Time only the named kernels (iterable of names; None: every kernel)."""
+    arg = ",".join(names).encode() if names else None
+    check(_lib.load().ipg_prof_select(_lib.default_context(), arg))

@@ -258,3 +258,63 @@ def virial_stress(s, p, precision="fp64", fused=True):
 def virial(s, p, **kw):
     """sum(virial_stress[1:3]) -- abstract_potential.jl:10."""
     return float(np.sum(virial_stress(s, p, **kw)[:3]))
+
+
+class VerletEvaluator:
+    """Energy, forces and virial of one potential over a device-resident
+    Verlet-skin list, for MD loops that move the atoms of one system: the
+    list holds every pair within rcut + skin of the positions at its last
+    build and is rebuilt on the GPU when an atom has moved more than skin/2
+    (the reference instead rebuilds the list on every call,
+    empirical_potential.jl:21).  skin in bohr."""
+
+    def __init__(self, s, p, skin, precision="fp64"):
+        from .device import DeviceArray, dev_system
+
+        L = _lib.load()
+        self._ctx = _lib.default_context()
+        self.table = s.unique_symbols()
+        sp = np.array([self.table.index(x) for x in s.symbols], dtype=np.int32)
+        terms = _terms_of(p)
+        self._arr = (_lib.PairTerm * len(terms))()
+        for k, (q, c) in enumerate(terms):
+            self._arr[k] = q._term(c, self.table)
+        self._nterms = len(terms)
+        self._prec = IPG_FP32 if precision in ("fp32", IPG_FP32) else IPG_FP64
+        self.n = len(s)
+        self.pos = DeviceArray.from_host(s.austrip_positions())
+        self._sp = DeviceArray.from_host(sp)
+        self._dsys = dev_system(self.pos, self._sp, self.n, s.austrip_box(), s.periodic)
+        self._e = DeviceArray((1,), np.float64)
+        self._f = DeviceArray((self.n, 3), np.float64)
+        self._v = DeviceArray((6,), np.float64)
+        h = C.c_void_p()
+        check(L.ipg_nlist_build_skin_dev(self._ctx, C.byref(self._dsys), p.get_rcutoff(), float(skin), 1,
+                                         C.byref(h)))
+        self._h = h
+        self.builds = 1
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                _lib.load().ipg_nlist_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+    def npairs(self):
+        n = C.c_int64()
+        check(_lib.load().ipg_nlist_info(self._h, None, C.byref(n), None, None))
+        return n.value
+
+    def step(self, positions):
+        """New positions (N x 3, bohr) -> (e, f, virial, rebuilt)."""
+        L = _lib.load()
+        self.pos.upload(np.ascontiguousarray(positions, dtype=np.float64).reshape(-1, 3))
+        r = C.c_int32()
+        check(L.ipg_nlist_update_dev(self._h, C.byref(self._dsys), C.byref(r)))
+        self.builds += r.value
+        check(L.ipg_pair_eval_dev(self._ctx, self._h, C.byref(self._dsys), C.cast(self._arr, C.c_void_p),
+                                  self._nterms, self._prec, C.c_void_p(self._e.ptr), C.c_void_p(self._f.ptr),
+                                  C.c_void_p(self._v.ptr)))
+        return float(self._e.download()[0]), self._f.download(), self._v.download(), bool(r.value)

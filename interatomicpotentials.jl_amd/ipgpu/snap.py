@@ -167,6 +167,54 @@ def compute_all_descriptors(s, snap, list_mode=None, want=("B", "dB", "W")):
             None if W is None else np.ascontiguousarray(W.T))
 
 
+BatchDescriptors = collections.namedtuple("BatchDescriptors", ["B", "dB", "erow", "W", "start"])
+
+
+def compute_batch_descriptors(systems, snap, precision="fp64", want=("B", "dB", "erow", "W")):
+    """Descriptors of a data set in one GPU call (a fitting loop over
+    snap.jl:103-124 and :226-285, CUTOFF list mode).  Returns packed arrays:
+    B (Ntot, ncoeff), dB (Ntot, 3, M), erow (nconf, M) = per-configuration
+    sums of B in species blocks (M = ncoeff*nspecies), W (nconf, 6, M) =
+    each configuration's virial descriptors as compute_all_descriptors
+    returns them, and start (nconf+1), the first packed atom of each
+    configuration.  precision 'fp32' runs U, Z
+    and the per-pair dB in FP32 with FP64 sums."""
+    L = _lib.load()
+    systems = list(systems)
+    if not systems:
+        raise ValueError("no configurations")
+    arr = (_lib.System * len(systems))()
+    keep = []
+    for c, s in enumerate(systems):
+        cs, k = make_csystem(s.positions, s.box, s.periodic, species_index(s.symbols, snap.species))
+        arr[c] = cs
+        keep.append(k)
+    natoms = np.array([a.natoms for a in arr], dtype=np.int64)
+    start = np.concatenate([[0], np.cumsum(natoms)]).astype(np.int64)
+    n, M = int(start[-1]), snap.ncoeff * len(snap.species)
+    B = np.zeros((n, snap.ncoeff)) if "B" in want else None
+    dB = np.zeros((n, M, 3)) if "dB" in want else None
+    erow = np.zeros((len(systems), M)) if "erow" in want else None
+    W = np.zeros((len(systems), M, 6)) if "W" in want else None
+    prec = IPG_FP32 if precision == "fp32" else IPG_FP64
+    check(L.ipg_snap_batch_descriptors(snap._h, len(systems), arr, prec, None if B is None else B.ctypes.data,
+                                       None if dB is None else dB.ctypes.data,
+                                       None if erow is None else erow.ctypes.data,
+                                       None if W is None else W.ctypes.data))
+    return BatchDescriptors(B, None if dB is None else np.ascontiguousarray(dB.transpose(0, 2, 1)), erow,
+                            None if W is None else np.ascontiguousarray(W.transpose(0, 2, 1)), start)
+
+
+def batch_info(snap, precision="fp64"):
+    """Launch shapes the batch autotuner chose and the captured-graph state."""
+    v = [C.c_int32() for _ in range(4)]
+    rep = C.c_int64()
+    check(_lib.load().ipg_snap_batch_info(snap._h, IPG_FP32 if precision == "fp32" else IPG_FP64,
+                                          *[C.byref(x) for x in v], C.byref(rep)))
+    return {"scan_lanes": v[0].value, "ui_atoms_per_wave": v[1].value, "dbdr_threads": v[2].value,
+            "graph": ("none", "ready", "disabled")[v[3].value], "replays": rep.value}
+
+
 class LBasisPotential:
     """linear_potential.jl:6-49: E = beta0 + sum_i beta_s(i) . B_i, F = dB . beta.
     beta has ncoeff entries (shared by every species, as the reference) or

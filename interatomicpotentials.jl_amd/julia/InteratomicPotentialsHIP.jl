@@ -293,6 +293,41 @@ function ChainRulesCore.rrule(::typeof(InteratomicPotentials.force), s::Abstract
     return F, force_pullback
 end
 
-export HIP
+"""
+    compute_batch_descriptors(systems, HIP(snap); precision=IPG_FP64)
+
+Descriptors of a data set in one GPU call (CUTOFF list mode), for fitting:
+per configuration `B` (`ncoeff x N_c`), `dB` (`3 x M x N_c`), the energy row
+(sum of B in species blocks, length `M`) and `W` (`6 x M`), as
+`compute_all_descriptors` returns them per configuration.
+"""
+function compute_batch_descriptors(systems::AbstractVector, h::HIP{<:SNAP}; precision=IPG_FP64)
+    snap = h.p
+    H = handle(snap)
+    M = H.ncoeff * length(snap.species)
+    nc = length(systems)
+    Xs = [positions_matrix(s, ustrip) for s in systems]
+    sps = [snap_species(s, snap) for s in systems]
+    sys = Vector{IPGSystem}(undef, nc)
+    n = sum(length, systems)
+    B = zeros(Float64, H.ncoeff, n)
+    dB = zeros(Float64, 3, M, n)
+    erow = zeros(Float64, M, nc)
+    W = zeros(Float64, 6, M, nc)
+    GC.@preserve Xs sps begin
+        for c in 1:nc
+            L, per = box_and_periodicity(systems[c], ustrip)
+            sys[c] = IPGSystem(length(systems[c]), pointer(Xs[c]), pointer(sps[c]), L, per)
+        end
+        check(ccall((:ipg_snap_batch_descriptors, libipgpu), Cint,
+                    (Ptr{Cvoid}, Int64, Ptr{IPGSystem}, Cint, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                    H.ptr, nc, sys, precision, B, dB, erow, W))
+    end
+    starts = cumsum([0; length.(systems)])
+    [(B=B[:, starts[c]+1:starts[c+1]], dB=dB[:, :, starts[c]+1:starts[c+1]], e=erow[:, c], W=W[:, :, c])
+     for c in 1:nc]
+end
+
+export HIP, compute_batch_descriptors
 
 end # module
