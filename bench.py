@@ -7,7 +7,13 @@ Step (value): one SNAP 2J=8 energy+force evaluation of an 8192-atom
 two-species bcc supercell (a = 3.16 A, seeded jitter 0.05 A, rcut = 5 A,
 switching on, beta ~ N(0, 0.1) of shape 2x55) through the FP32 adjoint path,
 inputs resident in HBM: SNAP neighbor list, U, Z/B/Y, per-pair dE/dr and
-the force gather.  Prints one JSON line.
+the force gather.  The timed region replays the step from its captured HIP
+graph K times, as an MD loop does, with HIP events around the dominant
+kernel only (found first from a kernel-by-kernel pass with events around
+every launch, reported as the per-kernel breakdown).  Also reported: the
+FP64 plain and adjoint paths, the LJ/Morse/ZBL pair kernel on 256k atoms,
+batched descriptors of 512 configurations (--batch-only runs just that),
+and the oracle on the host CPU.  Prints one JSON line.
 """
 import argparse
 import ctypes as C
@@ -165,6 +171,16 @@ def batch_bench(steps, warmup, nconfig, cpu_configs):
     res["kernels_fp32_ms"] = {k: v[1] for k, v in kernel_avg(
         ["batch_scan", "batch_list_finish", "snap_ui_fp32", "snap_zb_fp32", "snap_dbdr_fp32", "snap_db_gather",
          "batch_erow", "batch_w_atom", "batch_wrow", "scan", "fill_zero"]).items()}
+    from ipgpu import perfmodel
+
+    npairs = res["fp32"]["tuning"]["npairs"]
+    res["npairs"] = npairs
+    if "snap_dbdr_fp32" in res["kernels_fp32_ms"]:
+        ms = res["kernels_fp32_ms"]["snap_dbdr_fp32"]
+        fl = perfmodel.flops_dbdr_plain_per_pair(8) * npairs
+        res["dominant_kernel"] = {"kernel": "snap_dbdr_fp32", "ms": ms, "algorithmic_flops": fl,
+                                  "tflops": fl / (ms * 1e-3) / 1e12,
+                                  "frac_of_fp32_peak": fl / (ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS}
     if cpu_configs > 0:
         sys.path.insert(0, os.path.join(REPO, "oracle"))
         import oracle as O
@@ -312,6 +328,7 @@ def main():
     ap.add_argument("--no-lj", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-configs", type=int, default=512, help="configurations of the batched descriptor run")
+    ap.add_argument("--batch-only", action="store_true", help="run only the batched descriptor section")
     ap.add_argument("--cpu-sample", type=int, default=2048, help="atoms of the 1-thread CPU baseline")
     ap.add_argument("--probe", action="store_true",
                     help="short run for hardware-counter passes: 3 calibration copies of 256 MiB, then 3 uncaptured "
@@ -325,50 +342,35 @@ def main():
     import ipgpu
     from ipgpu import perfmodel
     from ipgpu._lib import IPG_SNAP_ADJOINT_FP32, IPG_SNAP_PLAIN_FP64, IPG_SNAP_ADJOINT_FP64
-    from ipgpu.device import prof_enable, prof_reset, set_graphs
+    from ipgpu.device import prof_enable, prof_reset, prof_select, set_graphs
 
     ipgpu.load()
     if a.probe:
         probe()
         return
+    if a.batch_only:
+        print(json.dumps({"batch": batch_bench(max(3, a.steps // 4), 1, a.batch_configs, 0)}))
+        return
     log("setup SNAP workload")
     s, sn, beta, sp, d, dsys = snap_setup()
     n = len(s)
 
-    # FP32 adjoint (value).  Timed region: K steps launched kernel by kernel
-    # on the context stream with HIP events around every launch (the
-    # per-kernel durations of the roofline come from these events; events
-    # inside a captured graph do not time its kernels).
+    # FP32 adjoint.  First a per-kernel breakdown: the step launched kernel
+    # by kernel (graphs off) with HIP events around every launch.
     step32 = lambda: snap_step(sn, dsys, d, IPG_SNAP_ADJOINT_FP32)
+    names32 = ["snap_ui_fp32", "snap_zy_fp32", "snap_dedr_fp32", "snap_force_gather", "snap_list_finish",
+               "snap_pairs", "nlist_count", "nlist_fill", "nlist_sort", "cell_assign", "cell_place", "cell_rank",
+               "fill_zero", "reduce_rows", "scan"]
     set_graphs(False)
     for _ in range(a.warmup):
         step32()
+    prof_select(None)
     prof_reset()
     prof_enable(True)
-    t32 = time_steps(step32, a.steps, 0)
+    t32_unc = time_steps(step32, a.steps, 0)
     prof_enable(False)
-    k32 = kernel_avg(["snap_ui_fp32", "snap_zy_fp32", "snap_dedr_fp32", "snap_force_gather", "snap_list_finish",
-                      "snap_pairs", "nlist_count", "nlist_fill", "nlist_sort", "cell_assign", "cell_place",
-                      "cell_rank", "fill_zero", "reduce_rows", "scan"])
-    f32 = d["f"].download()
-    e32 = float(d["e"].download()[0])
-    log("fp32 adjoint: %.3f ms/step (uncaptured, HIP events)" % (t32 * 1e3))
-    # the same step replayed from a captured HIP graph (no events)
-    set_graphs(True)
-    t32_graph = time_steps(step32, a.steps, max(2, a.warmup))
-    ginfo = sn.graph_info()
-    log("fp32 adjoint graph replay %.3f ms (%s)" % (t32_graph * 1e3, ginfo))
-    # FP64 plain (the reference's route) and FP64 adjoint, uncaptured
-    set_graphs(False)
-    steps64 = max(3, a.steps // 4)
-    t64 = time_steps(lambda: snap_step(sn, dsys, d, IPG_SNAP_PLAIN_FP64), steps64, 1)
-    f64 = d["f"].download()
-    e64 = float(d["e"].download()[0])
-    t64a = time_steps(lambda: snap_step(sn, dsys, d, IPG_SNAP_ADJOINT_FP64), steps64, 1)
-    f64a = d["f"].download()
-    set_graphs(True)
-    log("fp64 plain: %.3f ms/step, fp64 adjoint %.3f ms/step" % (t64 * 1e3, t64a * 1e3))
-    # roofline of the dominant FP32 kernel
+    k32 = kernel_avg(names32)
+    log("fp32 adjoint uncaptured with events on every kernel: %.3f ms/step" % (t32_unc * 1e3))
     nl = ipgpu.snap.snap_neighborlist(s, sn, list_mode="cutoff")
     npairs = len(nl.j)
     flops = {
@@ -377,8 +379,34 @@ def main():
         "snap_dedr_fp32": perfmodel.flops_dedr_per_pair(8) * npairs,
     }
     dom = max((k for k in flops if k in k32), key=lambda k: k32[k][1])
-    dom_ms = k32[dom][1]
+    # Timed region (value): K steps replayed from the captured HIP graph, as
+    # an MD loop runs them, with HIP events recorded (as graph nodes on the
+    # context stream) around the dominant kernel only; its average duration
+    # gives the roofline figure.
+    set_graphs(True)
+    for _ in range(max(2, a.warmup)):
+        step32()
+    prof_select([dom])
+    prof_reset()
+    prof_enable(True)
+    t32 = time_steps(step32, a.steps, 0)
+    prof_enable(False)
+    kdom = kernel_avg([dom])
+    prof_select(None)
+    dom_ms = kdom[dom][1]
     ach = flops[dom] / (dom_ms * 1e-3) / 1e12
+    f32 = d["f"].download()
+    e32 = float(d["e"].download()[0])
+    ginfo = sn.graph_info()
+    log("fp32 adjoint graph replay: %.3f ms/step, %s %.1f us (%s)" % (t32 * 1e3, dom, dom_ms * 1e3, ginfo))
+    # FP64 plain (the reference's route) and FP64 adjoint, graph replay
+    steps64 = max(3, a.steps // 4)
+    t64 = time_steps(lambda: snap_step(sn, dsys, d, IPG_SNAP_PLAIN_FP64), steps64, 2)
+    f64 = d["f"].download()
+    e64 = float(d["e"].download()[0])
+    t64a = time_steps(lambda: snap_step(sn, dsys, d, IPG_SNAP_ADJOINT_FP64), steps64, 2)
+    f64a = d["f"].download()
+    log("fp64 plain: %.3f ms/step, fp64 adjoint %.3f ms/step" % (t64 * 1e3, t64a * 1e3))
 
     # FP32 vs FP64 force errors (SNAP energies/forces are in the units of beta.B)
     df = f32 - f64
@@ -410,10 +438,10 @@ def main():
             "fp64_plain_atoms_per_s": n / t64,
             "fp64_adjoint_atoms_per_s": n / t64a,
             "speedup_fp32_adjoint_vs_fp64_plain": t64 / t32,
-            "kernels_fp32_ms": {k: v[1] for k, v in k32.items()},
+            "kernels_fp32_ms_uncaptured": {k: v[1] for k, v in k32.items()},
             "kernel_launches_per_step": {k: v[0] / a.steps for k, v in k32.items()},
-            "fp32_adjoint_ms_graph_replay": t32_graph * 1e3,
-            "fp32_adjoint_atoms_per_s_graph_replay": n / t32_graph,
+            "fp32_adjoint_ms_uncaptured_all_events": t32_unc * 1e3,
+            "timed_region": "graph replay, HIP events around %s only" % dom,
             "graph": ginfo,
             "fp32_vs_fp64_error": err,
         },

@@ -273,10 +273,11 @@ int ipg_snap_batch_descriptors_dev(ipg_snap* snap, int64_t nconf, const int64_t*
 /* Launch shapes the autotuner chose for a precision (0 before the first
  * call): lanes per atom of the neighbor scan, atoms per wave of the U
  * kernel and block size of the per-pair dB kernel; graph_state as in
- * ipg_snap_graph_info, and the graph replays so far.  No counterpart in the
- * reference. */
+ * ipg_snap_graph_info, the graph replays so far and the SNAP pair count of
+ * the last exact (uncaptured) batched run.  Any output may be NULL.  No
+ * counterpart in the reference. */
 int ipg_snap_batch_info(const ipg_snap* snap, int precision, int32_t* scan_lanes, int32_t* ui_atoms_per_wave,
-                        int32_t* dbdr_threads, int32_t* graph_state, int64_t* replays);
+                        int32_t* dbdr_threads, int32_t* graph_state, int64_t* replays, int64_t* npairs);
 /* Reverse mode of F(x, beta) for force matching (a ChainRulesCore rrule of
  * the force call): given the cotangent gF = dL/dF (3*natoms), returns
  * gbeta = gF^T dF/dbeta (nbeta entries, same layout as beta) and

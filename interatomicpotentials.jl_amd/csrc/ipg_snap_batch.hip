@@ -355,6 +355,7 @@ static void batch_descriptors(ipg_snap* sn, const BatchSys& b, int precision, co
   if (!tn.scan_lp && b.n > 0) batch_tune(sn, b, fp32, tn, flag);
   // exact run: the result of this call
   SnapListDev L = batch_run(sn, b, fp32, o, 0, 0, tn, flag);
+  sn->bpairs = L.list.npairs;
   if (!use_graph) return;
   graph_release(ctx, g);
   const int64_t np = L.list.npairs;
@@ -452,7 +453,7 @@ int ipg_snap_batch_descriptors(ipg_snap* sn, int64_t nconf, const ipg_system* sy
 }
 
 int ipg_snap_batch_info(const ipg_snap* sn, int precision, int32_t* scan_lanes, int32_t* ui_atoms_per_wave,
-                        int32_t* dbdr_threads, int32_t* graph_state, int64_t* replays) {
+                        int32_t* dbdr_threads, int32_t* graph_state, int64_t* replays, int64_t* npairs) {
   return guard([&] {
     IPG_REQUIRE(sn, "snap is NULL");
     IPG_REQUIRE(precision == IPG_FP32 || precision == IPG_FP64, "precision must be IPG_FP64 or IPG_FP32");
@@ -463,6 +464,7 @@ int ipg_snap_batch_info(const ipg_snap* sn, int precision, int32_t* scan_lanes, 
     const SnapGraph& g = sn->bgraph;
     if (graph_state) *graph_state = g.disabled[sn->ctx && sn->ctx->prof ? 1 : 0] ? 2 : (g.exec ? 1 : 0);
     if (replays) *replays = g.replays;
+    if (npairs) *npairs = sn->bpairs;
   });
 }
 

@@ -73,4 +73,5 @@ struct ipg_snap {
   ipg::BatchTune btune[2];     // per precision (FP64, FP32)
   ipg::BatchLayout blayout;
   int bprec = -1;              // precision the captured batch pipeline was built for
+  int64_t bpairs = 0;          // SNAP pairs of the last exact batched run
 };

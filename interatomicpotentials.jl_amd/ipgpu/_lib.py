@@ -136,7 +136,7 @@ def load():
             "ipg_nlist_build_skin_dev": (C.c_int, [P, pS, d, d, C.c_int, C.POINTER(P)]),
             "ipg_nlist_update_dev": (C.c_int, [P, pS, C.POINTER(i32)]),
             "ipg_snap_batch_info": (C.c_int, [P, C.c_int, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
-                                              C.POINTER(i32), C.POINTER(i64)]),
+                                              C.POINTER(i32), C.POINTER(i64), C.POINTER(i64)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

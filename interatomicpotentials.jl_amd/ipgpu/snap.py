@@ -208,11 +208,11 @@ def compute_batch_descriptors(systems, snap, precision="fp64", want=("B", "dB", 
 def batch_info(snap, precision="fp64"):
     """Launch shapes the batch autotuner chose and the captured-graph state."""
     v = [C.c_int32() for _ in range(4)]
-    rep = C.c_int64()
+    rep, npairs = C.c_int64(), C.c_int64()
     check(_lib.load().ipg_snap_batch_info(snap._h, IPG_FP32 if precision == "fp32" else IPG_FP64,
-                                          *[C.byref(x) for x in v], C.byref(rep)))
+                                          *[C.byref(x) for x in v], C.byref(rep), C.byref(npairs)))
     return {"scan_lanes": v[0].value, "ui_atoms_per_wave": v[1].value, "dbdr_threads": v[2].value,
-            "graph": ("none", "ready", "disabled")[v[3].value], "replays": rep.value}
+            "graph": ("none", "ready", "disabled")[v[3].value], "replays": rep.value, "npairs": npairs.value}
 
 
 class LBasisPotential:
