@@ -200,6 +200,80 @@ def batch_bench(steps, warmup, nconfig, cpu_configs):
     return res
 
 
+def small_configs_bench(steps):
+    """The remaining reference workloads, device-resident, one call per
+    step: LJ argon (1000-atom fcc, periodic, rcut 8.5 A) and the reference's
+    1000-atom LJ cluster (non-periodic, rcut 100 A: every pair), energy +
+    forces + virial with the list built in the call; SNAP 2J=6 on a
+    2000-atom bcc crystal (a = 3.32 A, rcut 4.67 A): plain FP64 energy +
+    forces in both list modes, and B + dB."""
+    import ipgpu
+    from ipgpu import workloads
+    from ipgpu._lib import IPG_FP64, IPG_SNAP_PLAIN_FP64, check
+    from ipgpu.device import DeviceArray, dev_system
+    from ipgpu.empirical import _terms_of
+
+    L = ipgpu.load()
+    ctx = ipgpu._lib.default_context()
+    res = {}
+
+    def pair_case(s, pot, name):
+        table = s.unique_symbols()
+        sp = np.array([table.index(x) for x in s.symbols], np.int32)
+        pos = DeviceArray.from_host(s.austrip_positions())
+        dsp = DeviceArray.from_host(sp)
+        dsys = dev_system(pos, dsp, len(s), s.austrip_box(), s.periodic)
+        terms = _terms_of(pot)
+        arr = (ipgpu._lib.PairTerm * len(terms))()
+        for k, (p, c) in enumerate(terms):
+            arr[k] = p._term(c, table)
+        e = DeviceArray((1,), np.float64)
+        f = DeviceArray((len(s), 3), np.float64)
+        v = DeviceArray((6,), np.float64)
+        fn = lambda: check(L.ipg_pair_energy_forces_dev(ctx, C.byref(dsys), C.cast(arr, C.c_void_p), len(terms),
+                                                        IPG_FP64, C.c_void_p(e.ptr), C.c_void_p(f.ptr),
+                                                        C.c_void_p(v.ptr)))
+        t = time_steps(fn, steps, 2)
+        nl = ipgpu.nnlist.neighborlist_arrays(s.austrip_positions(), s.austrip_box(), s.periodic, pot.get_rcutoff(),
+                                             want_disp=False)
+        res[name] = {"natoms": len(s), "half_pairs": int(len(nl[1])), "ms_per_call": t * 1e3,
+                     "pairs_per_s": len(nl[1]) / t, "energy_hartree": float(e.download()[0])}
+
+    s, pot = workloads.lj_argon_fcc()
+    pair_case(s, pot, "lj_argon_fcc_1000")
+    X = np.loadtxt(os.path.join(REPO, "tests", "golden", "lj_1000.xyz"))
+    cl = ipgpu.AtomsSystem(X, [8.0, 8.0, 8.0], (False,) * 3, ["Ar"] * len(X), unit="angstrom")
+    pair_case(cl, ipgpu.LennardJones(1.0 * ipgpu.eV, 1.0 * ipgpu.angstrom, 100.0 * ipgpu.angstrom, ["Ar"]),
+              "lj_cluster_1000_rcut100")
+
+    s = workloads.snap_bcc(ncell=10, a=3.32, jitter=0.03, nspecies=1, seed=3)
+    sn = ipgpu.snap.SNAP(len(s), 6, ["W"], 4.67, 0.0, 0.99363, [0.5], [1.0])
+    beta = DeviceArray.from_host(np.random.default_rng(33).normal(scale=0.1, size=sn.ncoeff))
+    pos = DeviceArray.from_host(s.positions)
+    dsys = dev_system(pos, None, len(s), s.box, s.periodic)
+    e = DeviceArray((1,), np.float64)
+    f = DeviceArray((len(s), 3), np.float64)
+    out = {"natoms": len(s), "twojmax": 6}
+    for mode, nm in ((1, "cutoff"), (0, "reference")):
+        fn = lambda: check(L.ipg_snap_energy_forces_dev(sn._h, C.byref(dsys), mode, C.c_void_p(beta.ptr), 0.0,
+                                                        IPG_SNAP_PLAIN_FP64, C.c_void_p(e.ptr), C.c_void_p(f.ptr)))
+        t = time_steps(fn, steps, 2)
+        out["plain_fp64_ms_%s" % nm] = t * 1e3
+        out["plain_fp64_atoms_per_s_%s" % nm] = len(s) / t
+    natoms = np.array([len(s)], np.int64)
+    box = np.array([s.box], np.float64)
+    per = np.ones((1, 3), np.int32)
+    B = DeviceArray((len(s), sn.ncoeff), np.float64)
+    dB = DeviceArray((len(s), sn.ncoeff, 3), np.float64)
+    fn = lambda: check(L.ipg_snap_batch_descriptors_dev(sn._h, 1, natoms.ctypes.data, box.ctypes.data,
+                                                        per.ctypes.data, C.c_void_p(pos.ptr), None, IPG_FP64,
+                                                        C.c_void_p(B.ptr), C.c_void_p(dB.ptr), None, None))
+    t = time_steps(fn, steps, 2)
+    out["B_dB_fp64_ms_cutoff"] = t * 1e3
+    res["snap_J6_bcc10_2000"] = out
+    return res
+
+
 def lj_bench(steps, warmup):
     import ipgpu
     from ipgpu import workloads, perfmodel
@@ -284,7 +358,7 @@ def probe():
     writes COPY_BYTES), so FETCH_SIZE / WRITE_SIZE of the SNAP kernels can be
     scaled by what the counters read for it."""
     import ipgpu
-    from ipgpu._lib import IPG_SNAP_ADJOINT_FP32, check, default_context
+    from ipgpu._lib import IPG_SNAP_ADJOINT_FP32, IPG_SNAP_PLAIN_FP64, check, default_context
     from ipgpu.device import DeviceArray, set_graphs, synchronize
 
     src = DeviceArray((COPY_BYTES // 8,), np.float64)
@@ -297,7 +371,13 @@ def probe():
     set_graphs(False)
     for _ in range(3):
         snap_step(sn, dsys, d, IPG_SNAP_ADJOINT_FP32)
+    snap_step(sn, dsys, d, IPG_SNAP_PLAIN_FP64)
     synchronize()
+    # one uncaptured FP32 batched-descriptor call over 32 configurations
+    from ipgpu import workloads
+
+    systems = workloads.snap_batch(nconfig=32, nmin=50, nmax=2000, rcut=5.0, seed=4)
+    ipgpu.snap.compute_batch_descriptors(systems, sn, precision="fp32")
     print(json.dumps({"probe": True, "copy_bytes": COPY_BYTES, "natoms": len(s)}))
 
 
@@ -329,10 +409,11 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-configs", type=int, default=512, help="configurations of the batched descriptor run")
     ap.add_argument("--batch-only", action="store_true", help="run only the batched descriptor section")
+    ap.add_argument("--lj-only", action="store_true", help="run only the pair-potential sections")
     ap.add_argument("--cpu-sample", type=int, default=2048, help="atoms of the 1-thread CPU baseline")
     ap.add_argument("--probe", action="store_true",
-                    help="short run for hardware-counter passes: 3 calibration copies of 256 MiB, then 3 uncaptured "
-                         "SNAP FP32 steps")
+                    help="short run for hardware-counter passes: 3 calibration copies of 256 MiB, 3 uncaptured "
+                         "SNAP FP32 steps, one FP64 plain step, one FP32 batched-descriptor call (32 configurations)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: every box here has one GPU and this engine has no multi-GPU path "
@@ -350,6 +431,10 @@ def main():
         return
     if a.batch_only:
         print(json.dumps({"batch": batch_bench(max(3, a.steps // 4), 1, a.batch_configs, 0)}))
+        return
+    if a.lj_only:
+        print(json.dumps({"lj": lj_bench(max(5, a.steps), a.warmup),
+                          "other_workloads": small_configs_bench(max(5, a.steps // 2))}))
         return
     log("setup SNAP workload")
     s, sn, beta, sp, d, dsys = snap_setup()
@@ -449,6 +534,9 @@ def main():
     if not a.no_lj:
         log("LJ/Morse/ZBL 256k pair kernel")
         out["lj"] = lj_bench(max(5, a.steps), a.warmup)
+    if not a.no_lj:
+        log("remaining workloads: LJ argon 1000, LJ cluster 1000, SNAP 2J=6 2000 atoms")
+        out["other_workloads"] = small_configs_bench(max(5, a.steps // 2))
     if not a.no_batch:
         log("batched SNAP descriptors")
         out["batch"] = batch_bench(max(3, a.steps // 4), 1, a.batch_configs, 0 if a.no_cpu else 2)

@@ -88,6 +88,8 @@ struct BatchSys {
 };
 // CUTOFF-mode SNAP list of packed configurations (rows sorted by j).  cap,
 // flag and ellK as build_snap_list; lp: lanes per atom (16, 32 or 64).
+// need: SNAPL_SYM gives rev (bit 4 of *flag marks a missing reverse pair),
+// SNAPL_REVERSE the general reverse rows.
 SnapListDev build_snap_list_batch(ipg_ctx* ctx, const BatchSys& b, const SnapCut& cut, int need, const char* tag,
                                   int64_t cap, int32_t* flag, int ellK, int lp);
 

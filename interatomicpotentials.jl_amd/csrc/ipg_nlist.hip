@@ -1007,6 +1007,7 @@ SnapListDev build_snap_list_batch(ipg_ctx* ctx, const BatchSys& b, const SnapCut
   out.rvec = scratch_t<double>(ctx, (t + "_rvec").c_str(), 3 * npa);
   out.R = (need & SNAPL_R) ? scratch_t<double>(ctx, (t + "_R").c_str(), npa) : nullptr;
   const int32_t* ell = scratch_t<int32_t>(ctx, (t + "_ell").c_str(), (size_t)std::max<int64_t>(n, 1) * K);
+  out.rev = (need & SNAPL_SYM) ? scratch_t<int32_t>(ctx, (t + "_rev").c_str(), npa) : nullptr;
   if (n > 0 && np > 0) {
     const int FL = K >= 64 ? 64 : 32;
     const size_t sm = sizeof(int32_t) * (size_t)(256 / FL) * K;
@@ -1015,13 +1016,15 @@ SnapListDev build_snap_list_batch(ipg_ctx* ctx, const BatchSys& b, const SnapCut
     launch(ctx, "batch_list_finish", [&] {
       if (FL == 64)
         k_snapc_finish<64><<<grid_for(n * 64, 256), 256, sm, ctx->stream>>>(
-            n, K, ell, cnt, L.off, b.pos, none, b.cfg, b.boxes, L.nbr, out.center, out.rvec, out.R, nullptr, flag);
+            n, K, ell, cnt, L.off, b.pos, none, b.cfg, b.boxes, L.nbr, out.center, out.rvec, out.R, out.rev,
+            flag);
       else
         k_snapc_finish<32><<<grid_for(n * 32, 256), 256, sm, ctx->stream>>>(
-            n, K, ell, cnt, L.off, b.pos, none, b.cfg, b.boxes, L.nbr, out.center, out.rvec, out.R, nullptr, flag);
+            n, K, ell, cnt, L.off, b.pos, none, b.cfg, b.boxes, L.nbr, out.center, out.rvec, out.R, out.rev,
+            flag);
     });
   }
-  if (need & (SNAPL_REVERSE | SNAPL_SYM)) {
+  if (need & SNAPL_REVERSE) {
     out.roff = scratch_t<int64_t>(ctx, (t + "_roff").c_str(), n + 1);
     out.rpair = scratch_t<int32_t>(ctx, (t + "_rpair").c_str(), npa);
     build_center_and_reverse(ctx, n, np, L.off, L.nbr, nullptr, out.roff, out.rpair, t.c_str());

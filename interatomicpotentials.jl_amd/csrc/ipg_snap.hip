@@ -368,6 +368,66 @@ __global__ void k_db_gather(SnapDev P, SnapRun R, const T* __restrict__ dblist, 
   }
 }
 
+// The same gather over a symmetric CUTOFF list: the pairs whose neighbor is
+// atom a are the reverse pairs rev[q] of a's own pairs q, met in the same
+// (ascending) order as in the general reverse rows, so the result equals
+// k_db_gather's bit for bit, without the reverse-row indirection.  With Wa,
+// the own-row pass also forms the per-atom virial rows of the own species
+// block (snap.jl:267-281, raw positions): Wa[a][k][v] for v = (x x, y y,
+// z z, y z, x z, x y), written by the lane whose first index matches.
+template <typename T>
+__global__ void k_db_gather_sym(SnapDev P, SnapRun R, const T* __restrict__ dblist, double* __restrict__ dB,
+                                const double* __restrict__ pos, double* __restrict__ Wa) {
+  const int64_t a = blockIdx.x;
+  const int M = P.ncoeff * P.nspecies, nc3 = 3 * P.ncoeff;
+  const int sa = R.species ? R.species[a] : 0;
+  const int64_t q0 = R.off[a], q1 = R.off[a + 1];
+  const double xa = Wa ? pos[3 * a] : 0.0, ya = Wa ? pos[3 * a + 1] : 0.0, za = Wa ? pos[3 * a + 2] : 0.0;
+  for (int t = threadIdx.x; t < nc3; t += blockDim.x) {
+    double own = 0.0, wx = 0.0, wy = 0.0, wz = 0.0;
+    if (Wa) {
+      for (int64_t q = q0; q < q1; q++) {
+        const int64_t j = R.nbr[q];
+        const double d = (double)dblist[q * nc3 + t];
+        own += d;
+        wx += d * xa - d * pos[3 * j];
+        wy += d * ya - d * pos[3 * j + 1];
+        wz += d * za - d * pos[3 * j + 2];
+      }
+      const int k = t / 3, d1 = t % 3;
+      double* w = Wa + (a * P.ncoeff + k) * 6;
+      if (d1 == 0) {
+        w[0] = wx;
+        w[5] = wy;
+        w[4] = wz;
+      } else if (d1 == 1) {
+        w[1] = wy;
+        w[3] = wz;
+      } else {
+        w[2] = wz;
+      }
+    } else {
+#pragma unroll 4
+      for (int64_t q = q0; q < q1; q++) own += (double)dblist[q * nc3 + t];
+    }
+    if (!dB) continue;
+    double acc[8];
+#pragma unroll
+    for (int s = 0; s < 8; s++) acc[s] = s == sa ? own : 0.0;
+#pragma unroll 4
+    for (int64_t q = q0; q < q1; q++) {
+      const int sc = R.species ? R.species[R.nbr[q]] : 0;
+      const double v = (double)dblist[(int64_t)R.rev[q] * nc3 + t];
+#pragma unroll
+      for (int s = 0; s < 8; s++)
+        if (s == sc) acc[s] -= v;
+    }
+#pragma unroll
+    for (int s = 0; s < 8; s++)
+      if (s < P.nspecies) dB[(a * M + s * P.ncoeff) * 3 + t] = acc[s];
+  }
+}
+
 __global__ void k_force_from_db(int64_t n, int M, const double* __restrict__ dB, const double* __restrict__ beta,
                                 double* __restrict__ F) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -473,7 +533,7 @@ static SnapCall snap_prepare(ipg_snap* sn, const DevSystem& s, int mode, const i
 // List bits an energy/force path needs: the adjoint gathers use the
 // reverse-pair index of the symmetric CUTOFF list, the plain path the
 // general reverse rows (its dB gather goes by the center's species).
-static int ef_need(int path) { return path == IPG_SNAP_PLAIN_FP64 ? SNAPL_REVERSE : SNAPL_SYM; }
+static int ef_need(int) { return SNAPL_SYM; }
 
 static void check_species_host(const ipg_snap* sn, const ipg_system* sys, const int32_t* appear) {
   for (int64_t i = 0; i < sys->natoms; i++) {
@@ -528,12 +588,25 @@ static double* run_dblist(ipg_snap* sn, const SnapCall& c, const double* Z) {
 }
 
 template <typename T>
-void snap_gather_db(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const T* dbl, double* dB) {
+void snap_gather_db(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const T* dbl, double* dB, const double* pos,
+                    double* Wa) {
   if (R.n == 0) return;
-  launch(ctx, "snap_db_gather", [&] { k_db_gather<T><<<(unsigned)R.n, 64, 0, ctx->stream>>>(P, R, dbl, dB); });
+  IPG_REQUIRE(R.rev || !Wa, "internal: the fused virial rows need the reverse-pair index");
+  // one lane per (coefficient, component), in whole waves
+  const int threads = (3 * P.ncoeff + 63) / 64 * 64;
+  if (R.rev)
+    launch(ctx, "snap_db_gather", [&] {
+      k_db_gather_sym<T><<<(unsigned)R.n, threads, 0, ctx->stream>>>(P, R, dbl, dB, pos, Wa);
+    });
+  else
+    launch(ctx, "snap_db_gather", [&] {
+      k_db_gather<T><<<(unsigned)R.n, threads, 0, ctx->stream>>>(P, R, dbl, dB);
+    });
 }
-template void snap_gather_db<float>(ipg_ctx*, const SnapDev&, const SnapRun&, const float*, double*);
-template void snap_gather_db<double>(ipg_ctx*, const SnapDev&, const SnapRun&, const double*, double*);
+template void snap_gather_db<float>(ipg_ctx*, const SnapDev&, const SnapRun&, const float*, double*, const double*,
+                                    double*);
+template void snap_gather_db<double>(ipg_ctx*, const SnapDev&, const SnapRun&, const double*, double*,
+                                     const double*, double*);
 
 static void gather_db(ipg_snap* sn, const SnapCall& c, const double* dbl, double* dB) {
   snap_gather_db<double>(sn->ctx, sn->dev, c.R, dbl, dB);
@@ -591,7 +664,7 @@ static void energy_forces_dev(ipg_snap* sn, const SnapCall& c, const double* d_b
 // A CUTOFF-mode list with a pair whose reverse is missing: rebuild the
 // general reverse rows and redo the adjoint force gather from the stored
 // dE_i/dr_ij of the same call.
-static void fix_asymmetric(ipg_snap* sn, SnapCall& c, double* d_forces) {
+static void fix_asymmetric(ipg_snap* sn, SnapCall& c, int path, const double* d_beta, double* d_forces) {
   ipg_ctx* ctx = sn->ctx;
   const int64_t n = c.R.n, np = c.L.list.npairs, npa = np > 0 ? np : 1;
   c.L.roff = scratch_t<int64_t>(ctx, "snap_list_roff", n + 1);
@@ -600,8 +673,19 @@ static void fix_asymmetric(ipg_snap* sn, SnapCall& c, double* d_forces) {
   c.R.roff = c.L.roff;
   c.R.rpair = c.L.rpair;
   c.R.rev = nullptr;
-  double* dedr = scratch_t<double>(ctx, "snap_dedr", 3 * std::max<int64_t>(np, 1));
   double* F = d_forces ? d_forces : scratch_t<double>(ctx, "snap_Fscratch", 3 * std::max<int64_t>(n, 1));
+  if (path == IPG_SNAP_PLAIN_FP64) {
+    // the dB gather again from the same per-pair dB, then F = dB . beta
+    const SnapDev& P = sn->dev;
+    const int M = P.ncoeff * P.nspecies;
+    const double* dbl = scratch_t<double>(ctx, "snap_dblist", 3 * (size_t)P.ncoeff * npa);
+    double* dB = scratch_t<double>(ctx, "snap_dB", 3 * (size_t)M * n);
+    snap_gather_db<double>(ctx, P, c.R, dbl, dB);
+    launch(ctx, "snap_force_from_db",
+           [&] { k_force_from_db<<<grid_for(3 * n, 256), 256, 0, ctx->stream>>>(n, M, dB, d_beta, F); });
+    return;
+  }
+  double* dedr = scratch_t<double>(ctx, "snap_dedr", 3 * std::max<int64_t>(np, 1));
   force_gather(ctx, c.R, dedr, F);
 }
 
@@ -724,7 +808,7 @@ static void snap_energy_forces_call(ipg_snap* sn, const DevSystem& s, int mode, 
     int32_t f = 0;
     d2h(ctx, &f, c.flag, sizeof(int32_t));
     sync(ctx);
-    if (f & 4) fix_asymmetric(sn, c, d_forces);
+    if (f & 4) fix_asymmetric(sn, c, path, d_beta, d_forces);
   }
   if (!use_graph || keep_graph) return;
   graph_release(ctx, g);

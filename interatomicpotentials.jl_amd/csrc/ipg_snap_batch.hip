@@ -22,8 +22,8 @@ namespace ipg {
 // out[c][s][e] = sum over the atoms a of configuration c with species s of
 // rows[a][e], e < width: the energy rows (rows = B, width ncoeff) and the
 // virial rows (rows = per-atom W, width 6 ncoeff).  One block per
-// configuration; for each chunk of 64 entries, lane e of each of the four
-// waves sums every fourth atom, then the four partial sums are added in wave
+// (configuration, chunk of 64 entries): lane e of each of the four waves
+// sums every fourth atom, then the four partial sums are added in wave
 // order (a fixed order, so repeated calls give identical rows).
 constexpr int ROWS_WAVES = 4;
 __global__ __launch_bounds__(64 * ROWS_WAVES) void k_batch_rows(const int64_t* __restrict__ start,
@@ -34,31 +34,28 @@ __global__ __launch_bounds__(64 * ROWS_WAVES) void k_batch_rows(const int64_t* _
   const int64_t c = blockIdx.x;
   const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int64_t a0 = start[c], a1 = start[c + 1];
-  for (int e0 = 0; e0 < width; e0 += 64) {
-    const int e = e0 + l;
-    double acc[8];
+  const int e = (int)blockIdx.y * 64 + l;
+  double acc[8];
 #pragma unroll
-    for (int s = 0; s < 8; s++) acc[s] = 0.0;
-    if (e < width)
-      for (int64_t a = a0 + w; a < a1; a += ROWS_WAVES) {
-        const int sa = species ? species[a] : 0;
-        const double v = rows[a * width + e];
+  for (int s = 0; s < 8; s++) acc[s] = 0.0;
+  if (e < width)
+    for (int64_t a = a0 + w; a < a1; a += ROWS_WAVES) {
+      const int sa = species ? species[a] : 0;
+      const double v = rows[a * width + e];
 #pragma unroll
-        for (int s = 0; s < 8; s++)
-          if (s == sa) acc[s] += v;
-      }
+      for (int s = 0; s < 8; s++)
+        if (s == sa) acc[s] += v;
+    }
 #pragma unroll
-    for (int s = 0; s < 8; s++) part[w][s][l] = acc[s];
-    __syncthreads();
-    if (w == 0 && e < width)
-      for (int s = 0; s < nspecies; s++) {
-        double v = part[0][s][l];
+  for (int s = 0; s < 8; s++) part[w][s][l] = acc[s];
+  __syncthreads();
+  if (w == 0 && e < width)
+    for (int s = 0; s < nspecies; s++) {
+      double v = part[0][s][l];
 #pragma unroll
-        for (int q = 1; q < ROWS_WAVES; q++) v += part[q][s][l];
-        out[(c * nspecies + s) * width + e] = v;
-      }
-    __syncthreads();
-  }
+      for (int q = 1; q < ROWS_WAVES; q++) v += part[q][s][l];
+      out[(c * nspecies + s) * width + e] = v;
+    }
 }
 
 // Per-atom virial rows of compute_all_descriptors (snap.jl:267-281), own
@@ -127,36 +124,44 @@ struct BatchOut {
 }  // namespace
 
 // Per-pair dB from Z, then the outputs built from it: the per-atom dB
-// gather and the per-configuration virial rows.
+// gather and the per-configuration virial rows.  Over a symmetric list
+// (R.rev) the gather forms the per-atom virial rows in the same pass.
 template <typename T>
 static void per_pair(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const BatchSys& b, const T* Z,
                      const BatchOut& o, const char* name) {
   if (!o.dB && !o.wrow) return;
   T* dbl = scratch_t<T>(ctx, name, 3 * (size_t)P.ncoeff * std::max<int64_t>(R.npairs, 1));
   snap_launch_dbdr<T>(ctx, P, R, Z, dbl);
-  if (o.dB) snap_gather_db<T>(ctx, P, R, dbl, o.dB);
-  if (o.wrow) {
-    double* Wa = scratch_t<double>(ctx, "batch_Wa", 6 * (size_t)P.ncoeff * std::max<int64_t>(b.n, 1));
-    if (b.n > 0)
+  double* Wa = o.wrow ? scratch_t<double>(ctx, "batch_Wa", 6 * (size_t)P.ncoeff * std::max<int64_t>(b.n, 1)) : nullptr;
+  if (R.rev) {
+    snap_gather_db<T>(ctx, P, R, dbl, o.dB, b.pos, Wa);
+  } else {
+    if (o.dB) snap_gather_db<T>(ctx, P, R, dbl, o.dB);
+    if (Wa && b.n > 0)
       launch(ctx, "batch_w_atom", [&] {
         k_batch_w_atom<T><<<(unsigned)b.n, 64, 0, ctx->stream>>>(P.ncoeff, R, b.pos, dbl, Wa);
       });
+  }
+  if (o.wrow) {
     launch(ctx, "batch_wrow", [&] {
-      k_batch_rows<<<(unsigned)b.nconf, 64 * ROWS_WAVES, 0, ctx->stream>>>(b.start, b.species, Wa, 6 * P.ncoeff,
-                                                                           P.nspecies, o.wrow);
+      k_batch_rows<<<dim3((unsigned)b.nconf, (6 * P.ncoeff + 63) / 64), 64 * ROWS_WAVES, 0, ctx->stream>>>(
+          b.start, b.species, Wa, 6 * P.ncoeff, P.nspecies, o.wrow);
     });
   }
 }
 
 // One pass of the pipeline.  cap/ellK > 0: capacity mode (for capture).
-// Returns the list (pair count and row width, for sizing the capture).
+// general: general reverse rows instead of the reverse-pair index (for a
+// list that a pair within an ulp of the cutoff made asymmetric).  Returns
+// the list (pair count and row width, for sizing the capture).
 static SnapListDev batch_run(ipg_snap* sn, const BatchSys& b, int fp32, const BatchOut& o, int64_t cap, int ellK,
-                             const BatchTune& tn, int32_t* flag) {
+                             const BatchTune& tn, int32_t* flag, bool general) {
   ipg_ctx* ctx = sn->ctx;
   const SnapDev& P = sn->dev;
   const SnapCut cut = snap_make_cut(sn->host, IPG_SNAP_LIST_CUTOFF);
   zero(ctx, flag, sizeof(int32_t));
-  SnapListDev L = build_snap_list_batch(ctx, b, cut, SNAPL_REVERSE, "snap_batch", cap, flag, ellK, tn.scan_lp);
+  SnapListDev L = build_snap_list_batch(ctx, b, cut, general ? SNAPL_REVERSE : SNAPL_SYM, "snap_batch", cap, flag,
+                                        ellK, tn.scan_lp);
   SnapRun R;
   R.n = b.n;
   R.npairs = L.list.npairs;
@@ -168,6 +173,7 @@ static SnapListDev batch_run(ipg_snap* sn, const BatchSys& b, int fp32, const Ba
   R.species = b.species;
   R.roff = L.roff;
   R.rpair = L.rpair;
+  R.rev = L.rev;
   R.apw = tn.ui_apw;
   R.dbdr_threads = tn.dbdr_threads;
   const int64_t n = std::max<int64_t>(b.n, 1);
@@ -188,8 +194,8 @@ static SnapListDev batch_run(ipg_snap* sn, const BatchSys& b, int fp32, const Ba
   }
   if (o.erow)
     launch(ctx, "batch_erow", [&] {
-      k_batch_rows<<<(unsigned)b.nconf, 64 * ROWS_WAVES, 0, ctx->stream>>>(b.start, b.species, Bv, P.ncoeff,
-                                                                           P.nspecies, o.erow);
+      k_batch_rows<<<dim3((unsigned)b.nconf, (P.ncoeff + 63) / 64), 64 * ROWS_WAVES, 0, ctx->stream>>>(
+          b.start, b.species, Bv, P.ncoeff, P.nspecies, o.erow);
     });
   return L;
 }
@@ -205,13 +211,13 @@ static void batch_tune(ipg_snap* sn, const BatchSys& b, int fp32, BatchTune& tn,
   for (int k = 0; k < 3; k++) {
     zero(ctx, flag, sizeof(int32_t));
     tn.ms_scan[k] = time_named(ctx, "batch_scan", [&] {
-      build_snap_list_batch(ctx, b, cut, SNAPL_REVERSE, "snap_batch", 0, flag, 0, lps[k]);
+      build_snap_list_batch(ctx, b, cut, SNAPL_SYM, "snap_batch", 0, flag, 0, lps[k]);
     });
     if (tn.ms_scan[k] < tn.ms_scan[best]) best = k;
   }
   tn.scan_lp = lps[best];
   zero(ctx, flag, sizeof(int32_t));
-  SnapListDev L = build_snap_list_batch(ctx, b, cut, SNAPL_REVERSE, "snap_batch", 0, flag, 0, tn.scan_lp);
+  SnapListDev L = build_snap_list_batch(ctx, b, cut, SNAPL_SYM, "snap_batch", 0, flag, 0, tn.scan_lp);
   SnapRun R;
   R.n = b.n;
   R.npairs = L.list.npairs;
@@ -353,8 +359,17 @@ static void batch_descriptors(ipg_snap* sn, const BatchSys& b, int precision, co
     g.overflows++;
   }
   if (!tn.scan_lp && b.n > 0) batch_tune(sn, b, fp32, tn, flag);
-  // exact run: the result of this call
-  SnapListDev L = batch_run(sn, b, fp32, o, 0, 0, tn, flag);
+  // exact run: the result of this call (again with the general reverse rows
+  // when a pair within an ulp of the cutoff left the list asymmetric)
+  bool general = false;
+  SnapListDev L = batch_run(sn, b, fp32, o, 0, 0, tn, flag, false);
+  int32_t f = 0;
+  d2h(ctx, &f, flag, sizeof(int32_t));
+  sync(ctx);
+  if (f & 4) {
+    general = true;
+    L = batch_run(sn, b, fp32, o, 0, 0, tn, flag, true);
+  }
   sn->bpairs = L.list.npairs;
   if (!use_graph) return;
   graph_release(ctx, g);
@@ -362,7 +377,7 @@ static void batch_descriptors(ipg_snap* sn, const BatchSys& b, int precision, co
   g.capB = np + np / 8 + 256;
   g.ellK = L.ellK;
   g.flag = flag;
-  auto step = [&] { batch_run(sn, b, fp32, o, g.capB, g.ellK, tn, flag); };
+  auto step = [&] { batch_run(sn, b, fp32, o, g.capB, g.ellK, tn, flag, general); };
   if (!graph_capture(ctx, g, step)) return;
   sn->bprec = precision;
   g.n = b.n;

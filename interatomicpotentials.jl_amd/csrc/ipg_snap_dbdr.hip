@@ -90,7 +90,7 @@ __device__ __forceinline__ S pick3(const S (&v)[3], int k) {
 constexpr int DBDR_THREADS = 128;
 
 template <int J, typename S>
-__global__ __launch_bounds__(DBDR_THREADS, 1) void k_snap_dbdr(SnapDev P, SnapRun R, const S* __restrict__ Zg,
+__global__ __launch_bounds__(DBDR_THREADS, sizeof(S) == 4 ? 3 : 1) void k_snap_dbdr(SnapDev P, SnapRun R, const S* __restrict__ Zg,
                                                             S* __restrict__ dblist) {
   constexpr int NZ = dbdr::idxz_max(J);
   constexpr int NB = dbdr::ncoeff(J);

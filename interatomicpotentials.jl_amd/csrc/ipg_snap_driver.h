@@ -40,8 +40,11 @@ struct SnapGraph {
 
 // Shared by the single-system and batched drivers (ipg_snap.hip).
 SnapCut snap_make_cut(const SnapHost& h, int mode);
+// dB per atom from the per-pair dB (dB may be NULL); over a symmetric list
+// (R.rev) optionally also the per-atom virial rows Wa [n][ncoeff][6].
 template <typename T>
-void snap_gather_db(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const T* dbl, double* dB);
+void snap_gather_db(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const T* dbl, double* dB,
+                    const double* pos = nullptr, double* Wa = nullptr);
 bool graph_capture(ipg_ctx* ctx, SnapGraph& g, const std::function<void()>& step);
 void graph_release(ipg_ctx* ctx, SnapGraph& g);
 
