@@ -33,8 +33,12 @@ DevSystem upload_system(ipg_ctx* ctx, const ipg_system* s, const char* tag);  //
 // Builds a list with membership PeriodicEuclidean distance <= rcut.
 // Storage comes from context scratch buffers named after `tag`.  sorted:
 // j ascending per atom.  want_disp: store get_displacement(x_i, x_j).
+// ellK != 0 (unsorted lists, with capflag): the scan writes each atom's
+// neighbors into rows of width ellK (-1: from the density) in the order the
+// CSR list would hold them; when no row overflowed the list comes back with
+// ellK > 0, the rows in ell and their lengths in cnt, and nbr unfilled.
 DevList build_list(ipg_ctx* ctx, const DevSystem& s, double rcut, int kind, bool sorted, bool want_disp,
-                   const char* tag, int64_t cap = 0, int32_t* capflag = nullptr);
+                   const char* tag, int64_t cap = 0, int32_t* capflag = nullptr, int ellK = 0);
 
 // Same, into caller-owned device buffers allocated by the function (for
 // ipg_nlist objects); frees nothing.

@@ -496,7 +496,7 @@ static DevList build_list_impl(ipg_ctx* ctx, const DevSystem& s, double rcut, in
     const double expect = (double)s.n / vol * 4.18879020478639 * rcut * rcut * rcut;
     ellK = (int)std::min<double>(4096.0, std::ceil((1.5 * expect + 32.0) / 32.0) * 32.0);
   }
-  const bool ell_mode = ellK > 0 && sc && sorted;
+  const bool ell_mode = ellK > 0 && (sc ? sorted : !sorted);
   L.ellK_req = ell_mode ? ellK : 0;
   int32_t* ell = ell_mode ? scratch_t<int32_t>(ctx, "nl_ell", (size_t)s.n * ellK) : nullptr;
   auto scan = [&](auto lp, auto mode, int64_t* offp, int32_t* nbrp) {
@@ -571,10 +571,12 @@ static DevList build_list_impl(ipg_ctx* ctx, const DevSystem& s, double rcut, in
 }
 
 DevList build_list(ipg_ctx* ctx, const DevSystem& s, double rcut, int kind, bool sorted, bool want_disp,
-                   const char* tag, int64_t cap, int32_t* capflag) {
+                   const char* tag, int64_t cap, int32_t* capflag, int ellK) {
   std::string t(tag);
-  return build_list_impl(ctx, s, rcut, kind, sorted, want_disp, cap, capflag, nullptr,
-                         [&](const char* what, size_t bytes) { return scratch(ctx, (t + "_" + what).c_str(), bytes); });
+  IPG_REQUIRE(ellK == 0 || (!sorted && capflag), "internal: row-buffer lists are unsorted and need a flag");
+  return build_list_impl(
+      ctx, s, rcut, kind, sorted, want_disp, cap, capflag, nullptr,
+      [&](const char* what, size_t bytes) { return scratch(ctx, (t + "_" + what).c_str(), bytes); }, ellK);
 }
 
 DevList build_list_owned(ipg_ctx* ctx, const DevSystem& s, double rcut, int kind, bool sorted, bool want_disp) {
@@ -1042,7 +1044,13 @@ __global__ void k_max_disp2(int64_t n, const double* __restrict__ pos, const dou
     m = fmax(m, ipg_pe_sumsq(box, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], ref[3 * i], ref[3 * i + 1],
                              ref[3 * i + 2]));
   for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
-  if ((threadIdx.x & 63) == 0) atomicMax((unsigned long long*)out, (unsigned long long)__double_as_longlong(m));
+  __shared__ double wmax[4];  // blockDim 256: one atomic per block
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    m = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
+    atomicMax((unsigned long long*)out, (unsigned long long)__double_as_longlong(m));
+  }
 }
 
 // (Re)builds a skin list at cut + skin from the system's current positions
@@ -1168,8 +1176,8 @@ int ipg_nlist_update_dev(ipg_nlist* list, const ipg_system* sys, int32_t* rebuil
       const double half = 0.5 * list->skin - 1e-9 * list->rcut;
       zero(ctx, list->dmax, sizeof(uint64_t));
       launch(ctx, "nlist_max_disp", [&] {
-        k_max_disp2<<<std::min<unsigned>(grid_for(s.n, 256), 1024), 256, 0, ctx->stream>>>(s.n, s.pos, list->ref,
-                                                                                           s.box, list->dmax);
+        k_max_disp2<<<std::min<unsigned>(grid_for(s.n, 256), 512), 256, 0, ctx->stream>>>(s.n, s.pos, list->ref,
+                                                                                          s.box, list->dmax);
       });
       uint64_t bits = 0;
       d2h(ctx, &bits, list->dmax, sizeof(uint64_t));

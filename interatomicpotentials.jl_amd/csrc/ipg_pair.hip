@@ -101,10 +101,13 @@ __device__ __forceinline__ T wave_sum_lanes(T v, int lanes) {
   return v;
 }
 
-// One block: 256 threads; LANES lanes per atom.
-template <typename T, int LANES>
+// One block: 256 threads; LANES lanes per atom.  ROWS: the neighbors of
+// atom a are rows[a*K .. a*K + cnt[a]) (the list's rows before compaction,
+// in the CSR order), else nbr[off[a] .. off[a+1]).
+template <typename T, int LANES, bool ROWS>
 __global__ __launch_bounds__(256) void k_pair(int64_t n, const int64_t* __restrict__ off,
-                                              const int32_t* __restrict__ nbr, const double* __restrict__ pos,
+                                              const int32_t* __restrict__ nbr, const int32_t* __restrict__ cnt,
+                                              int K, const double* __restrict__ pos,
                                               const int32_t* __restrict__ species, IpgBox box, PairTable tab,
                                               double* __restrict__ forces, double* __restrict__ part) {
   const int tid = threadIdx.x;
@@ -115,8 +118,8 @@ __global__ __launch_bounds__(256) void k_pair(int64_t n, const int64_t* __restri
   if (a < n) {
     const double xa = pos[3 * a], ya = pos[3 * a + 1], za = pos[3 * a + 2];
     const int sa = species ? species[a] : 0;
-    const int64_t q1 = off[a + 1];
-    for (int64_t q = off[a] + lane; q < q1; q += LANES) {
+    const int64_t q0 = ROWS ? a * K : off[a], q1 = ROWS ? a * K + cnt[a] : off[a + 1];
+    for (int64_t q = q0 + lane; q < q1; q += LANES) {
       const int32_t j = nbr[q];
       const double xj = pos[3 * j], yj = pos[3 * j + 1], zj = pos[3 * j + 2];
       const int sj = species ? species[j] : 0;
@@ -273,8 +276,12 @@ static void run_pair(ipg_ctx* ctx, const DevSystem& s, const DevList& L, const P
     constexpr int LN = decltype(lanes_c)::value;
     const char* nm = std::is_same<T, double>::value ? "pair_fp64" : "pair_fp32";
     launch(ctx, nm, [&] {
-      k_pair<T, LN><<<(unsigned)nblk, 256, 0, ctx->stream>>>(s.n, L.off, L.nbr, s.pos, s.species, s.box, tab,
-                                                             d_forces, part);
+      if (L.ellK > 0)
+        k_pair<T, LN, true><<<(unsigned)nblk, 256, 0, ctx->stream>>>(s.n, nullptr, L.ell, L.cnt, L.ellK, s.pos,
+                                                                     s.species, s.box, tab, d_forces, part);
+      else
+        k_pair<T, LN, false><<<(unsigned)nblk, 256, 0, ctx->stream>>>(s.n, L.off, L.nbr, nullptr, 0, s.pos,
+                                                                      s.species, s.box, tab, d_forces, part);
     });
   };
   if (precision == IPG_FP64) {
@@ -300,7 +307,11 @@ static void pair_dev(ipg_ctx* ctx, const DevSystem& s, const ipg_pair_term* term
     L = *prebuilt;
     IPG_REQUIRE(L.n == s.n, "list natoms does not match the system");
   } else {
-    L = build_list(ctx, s, rc, KIND_FULL, false, false, "pair_list");
+    // one scan into neighbor rows (no second, filling scan) when no row
+    // outgrows the width estimated from the density
+    int32_t* flag = scratch_t<int32_t>(ctx, "pair_list_flag", 1);
+    zero(ctx, flag, sizeof(int32_t));
+    L = build_list(ctx, s, rc, KIND_FULL, false, false, "pair_list", 0, flag, -1);
   }
   // terms whose rcut is below the list's test their own membership
   PairTable tab = prepare_terms(terms, nterms, list_rcut);
