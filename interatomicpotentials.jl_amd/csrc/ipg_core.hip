@@ -337,8 +337,51 @@ __global__ __launch_bounds__(RED1_BLOCK) void k_reduce_rows_single(const double*
   }
 }
 
+// Narrow rows (width <= 8) in one pass: every thread sums its strided rows
+// for all columns at once, then fixed butterflies inside each wave and the
+// wave partials in wave order.
+template <int W>
+__global__ __launch_bounds__(RED1_BLOCK) void k_reduce_rows_narrow(const double* __restrict__ rows, int64_t n,
+                                                                   double* __restrict__ out, int accumulate,
+                                                                   double offset) {
+  __shared__ double sh[RED1_BLOCK / 64][W];
+  double acc[W];
+#pragma unroll
+  for (int w = 0; w < W; w++) acc[w] = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += RED1_BLOCK)
+#pragma unroll
+    for (int w = 0; w < W; w++) acc[w] += rows[i * W + w];
+#pragma unroll
+  for (int w = 0; w < W; w++) {
+    for (int o = 32; o > 0; o >>= 1) acc[w] += __shfl_xor(acc[w], o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6][w] = acc[w];
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < W) {
+    double t = 0.0;
+    for (int k = 0; k < RED1_BLOCK / 64; k++) t += sh[k][threadIdx.x];
+    out[threadIdx.x] = offset + (accumulate ? out[threadIdx.x] + t : t);
+  }
+}
+
 void reduce_rows(ipg_ctx* ctx, const double* d_rows, int64_t n, int width, double* d_out, bool accumulate,
                  double offset) {
+  if (n <= 64 * RED1_BLOCK && width >= 1 && width <= 8) {
+    launch(ctx, "reduce_rows", [&] {
+      const int acc = accumulate ? 1 : 0;
+      switch (width) {
+        case 1: k_reduce_rows_narrow<1><<<1, RED1_BLOCK, 0, ctx->stream>>>(d_rows, n, d_out, acc, offset); break;
+        case 2: k_reduce_rows_narrow<2><<<1, RED1_BLOCK, 0, ctx->stream>>>(d_rows, n, d_out, acc, offset); break;
+        case 3: k_reduce_rows_narrow<3><<<1, RED1_BLOCK, 0, ctx->stream>>>(d_rows, n, d_out, acc, offset); break;
+        case 4: k_reduce_rows_narrow<4><<<1, RED1_BLOCK, 0, ctx->stream>>>(d_rows, n, d_out, acc, offset); break;
+        case 5: k_reduce_rows_narrow<5><<<1, RED1_BLOCK, 0, ctx->stream>>>(d_rows, n, d_out, acc, offset); break;
+        case 6: k_reduce_rows_narrow<6><<<1, RED1_BLOCK, 0, ctx->stream>>>(d_rows, n, d_out, acc, offset); break;
+        case 7: k_reduce_rows_narrow<7><<<1, RED1_BLOCK, 0, ctx->stream>>>(d_rows, n, d_out, acc, offset); break;
+        default: k_reduce_rows_narrow<8><<<1, RED1_BLOCK, 0, ctx->stream>>>(d_rows, n, d_out, acc, offset); break;
+      }
+    });
+    return;
+  }
   if (n <= 64 * RED1_BLOCK) {
     launch(ctx, "reduce_rows", [&] {
       k_reduce_rows_single<<<1, RED1_BLOCK, 0, ctx->stream>>>(d_rows, n, width, d_out, accumulate ? 1 : 0, offset);

@@ -524,16 +524,20 @@ static DevList build_list_impl(ipg_ctx* ctx, const DevSystem& s, double rcut, in
   // exact size (reads the total back) or a capacity given by the caller
   // (no host synchronization; the device total is L.off[n], clamped to the
   // capacity, and capflag records an overflow)
-  int64_t np = cap > 0 ? cap : exclusive_scan(ctx, cnt, s.n, L.off, true);
+  int64_t np = cap;
   if (cap > 0) exclusive_scan(ctx, cnt, s.n, L.off, false, cap, capflag, 1);
+  else if (ell_mode) exclusive_scan(ctx, cnt, s.n, L.off, false);  // total read with the flag below
+  else np = exclusive_scan(ctx, cnt, s.n, L.off, true);
   L.exact = cap <= 0;
   if (ell_mode) {
     // The rows are used unless one overflowed.  With a capacity the
     // overflow bit makes the caller recompute; with exact sizing it is read
-    // here, and the list is recounted and filled without the row buffer.
+    // here (with the total: one synchronization), and the list is recounted
+    // and filled without the row buffer.
     bool ok = true;
     if (cap <= 0) {
       int32_t f = 0;
+      d2h(ctx, &np, L.off + s.n, sizeof(int64_t));
       d2h(ctx, &f, capflag, sizeof(int32_t));
       sync(ctx);
       ok = (f & 8) == 0;
