@@ -89,8 +89,11 @@ __device__ __forceinline__ S pick3(const S (&v)[3], int k) {
 
 constexpr int DBDR_THREADS = 128;
 
+// Occupancy: 3 waves per SIMD in FP32 and 2 in FP64.  A few accumulators
+// then spill to scratch, which measured faster than the spill-free 2 and 1
+// waves (and than 4 and 3, where the spills grow).
 template <int J, typename S>
-__global__ __launch_bounds__(DBDR_THREADS, sizeof(S) == 4 ? 3 : 1) void k_snap_dbdr(SnapDev P, SnapRun R, const S* __restrict__ Zg,
+__global__ __launch_bounds__(DBDR_THREADS, sizeof(S) == 4 ? 3 : 2) void k_snap_dbdr(SnapDev P, SnapRun R, const S* __restrict__ Zg,
                                                             S* __restrict__ dblist) {
   constexpr int NZ = dbdr::idxz_max(J);
   constexpr int NB = dbdr::ncoeff(J);
