@@ -485,7 +485,7 @@ static DevList build_list_impl(ipg_ctx* ctx, const DevSystem& s, double rcut, in
   });
   // lanes per atom from the expected candidates per contiguous row range
   const double cand = (double)s.n / (double)g.ncell * (g.full[0] ? g.nc[0] : std::min(3, g.nc[0]));
-  const int LP = cand >= 40 ? 64 : (cand >= 18 ? 32 : (cand >= 9 ? 16 : 8));
+  const int LP = cand >= 80 ? 64 : (cand >= 18 ? 32 : (cand >= 9 ? 16 : 8));
   const unsigned gl = grid_for(s.n * LP, 256);
   SnapCutDev scd{};
   if (sc) scd = *sc;

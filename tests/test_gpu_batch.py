@@ -181,3 +181,34 @@ def test_batch_device_entry_point(ipg):
         assert np.array_equal(dB.download().transpose(0, 2, 1), ref.dB)
         assert np.array_equal(er.download(), ref.erow)
         assert np.array_equal(W.download().transpose(0, 2, 1), ref.W)
+
+
+def test_batch_device_species_null_is_species_zero(ipg):
+    """species NULL on the device entry point means every atom has species 0."""
+    from ipgpu import _lib
+    from ipgpu.device import DeviceArray
+
+    from ipgpu.system import AtomsSystem
+
+    g = ipg.snap.SNAP(100, 6, ["W"], 2.5, 0.0, 0.99363, [1.0], [1.0], switch_flag=True, list_mode="cutoff")
+    systems = [AtomsSystem(s.positions, s.box, s.periodic, ["W"] * len(s), unit="angstrom")
+               for s in configs(ipg, n=3, nmin=30, nmax=90, seed=81)]
+    n = sum(len(s) for s in systems)
+    pos = DeviceArray.from_host(np.concatenate([s.positions for s in systems]))
+    zeros = DeviceArray.from_host(np.zeros(n, np.int32))
+    natoms = np.array([len(s) for s in systems], dtype=np.int64)
+    box = np.array([s.box for s in systems], dtype=np.float64)
+    per = np.ones((len(systems), 3), dtype=np.int32)
+    L = _lib.load()
+    out = []
+    for sp in (None, C.c_void_p(zeros.ptr)):
+        B = DeviceArray((n, g.ncoeff), np.float64)
+        dB = DeviceArray((n, g.ncoeff, 3), np.float64)
+        _lib.check(L.ipg_snap_batch_descriptors_dev(g._h, len(systems), natoms.ctypes.data, box.ctypes.data,
+                                                    per.ctypes.data, C.c_void_p(pos.ptr), sp, _lib.IPG_FP64,
+                                                    C.c_void_p(B.ptr), C.c_void_p(dB.ptr), None, None))
+        _lib.check(L.ipg_ctx_synchronize(_lib.default_context()))
+        out.append((B.download(), dB.download()))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    Bs = np.concatenate([ipg.snap.compute_local_descriptors(s, g, list_mode="cutoff") for s in systems])
+    assert rel(out[0][0], Bs) < 1e-12

@@ -115,3 +115,33 @@ def test_graph_with_profiling_events(ipg):
         assert n == 3 and ms > 0.0, (n, ms)
     finally:
         device.prof_enable(False)
+
+
+def test_prof_select_times_only_the_named_kernels(ipg):
+    """Events only around the selected kernel, captured and uncaptured; the
+    results are those of the unprofiled step."""
+    from ipgpu import device
+
+    s, sn, d, dsys = setup(ipg, seed=14)
+    path = "adjoint_fp32"
+    ref = uncaptured(ipg, sn, d, dsys, path)
+    try:
+        device.prof_select(["snap_zy_fp32"])
+        device.prof_enable(True)
+        for graphs in (False, True):
+            device.set_graphs(graphs)
+            device.prof_reset()
+            for _ in range(3):
+                assert same(step(ipg, sn, d, dsys, path), ref)
+            n, ms = device.prof_query("snap_zy_fp32")
+            assert n == 3 and ms > 0.0, (graphs, n, ms)
+            assert device.prof_query("snap_ui_fp32")[0] == 0
+            assert device.prof_query("snap_dedr_fp32")[0] == 0
+        device.prof_select(None)
+        device.prof_reset()
+        assert same(step(ipg, sn, d, dsys, path), ref)
+        assert device.prof_query("snap_ui_fp32")[0] == 1
+    finally:
+        device.prof_enable(False)
+        device.prof_select(None)
+        device.set_graphs(True)
