@@ -405,6 +405,55 @@ __device__ __forceinline__ void transpose_sum(T* v, int l) {
   }
 }
 
+// The same transposed sum for float values without the LDS crossbar:
+// exchanges at distance 32 and 16 are v_permlane32_swap / v_permlane16_swap
+// of the two halves (after the swap, keep + partner's send is simply the
+// sum of the two registers in every lane), shorter distances read the
+// partner through DPP.  The DPP partner is a mirror within 16 or 8 lanes
+// or a quad permutation: it differs from the lane in bit D and agrees on
+// the higher bits, so each stage still adds two disjoint lane sets holding
+// the same values, and lane l ends with the same value as in transpose_sum.
+template <int D>
+__device__ __forceinline__ float dpp_partner(float x) {
+  static_assert(D == 8 || D == 4 || D == 2 || D == 1, "DPP partner distance");
+  // row_mirror (15 - l), row_half_mirror (7 - l), quad_perm [2,3,0,1], [1,0,3,2]
+  constexpr int ctrl = D == 8 ? 0x140 : D == 4 ? 0x141 : D == 2 ? 0x4E : 0xB1;
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), ctrl, 0xF, 0xF, false));
+}
+template <int D>
+__device__ __forceinline__ float swap_sum(float a, float b) {
+  if constexpr (D == 32) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+  } else {
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+  }
+}
+template <int M, int D>
+__device__ __forceinline__ void transpose_sum_f32(float* v, int l) {
+  if constexpr (D >= 1) {
+    if constexpr (M > 1) {
+      static_for<0, M / 2>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        if constexpr (D >= 16) {
+          v[k] = swap_sum<D>(v[k], v[k + M / 2]);
+        } else {
+          const bool hi = (l & D) != 0;
+          const float send = hi ? v[k] : v[k + M / 2];
+          const float keep = hi ? v[k + M / 2] : v[k];
+          v[k] = keep + dpp_partner<D>(send);
+        }
+      });
+      transpose_sum_f32<M / 2, D / 2>(v, l);
+    } else {
+      if constexpr (D >= 16) v[0] = swap_sum<D>(v[0], v[0]);
+      else v[0] = v[0] + dpp_partner<D>(v[0]);
+      transpose_sum_f32<1, D / 2>(v, l);
+    }
+  }
+}
+
 // Components of a U value for the lane sums: (re, im), and for Dual runs
 // the tangents after them.
 template <typename S> struct UComp { static constexpr int NC = 2; };
@@ -486,7 +535,8 @@ __global__ __launch_bounds__(64) void k_snap_ui(SnapDev P, SnapRun R, S* __restr
           static_for<e + 1, B>([&](auto ec) {
             static_for<0, NC>([&](auto cc) { buf[decltype(cc)::value * B + decltype(ec)::value] = (T)0; });
           });
-        transpose_sum<NV, G / 2>(buf, l);
+        if constexpr (std::is_same<T, float>::value) transpose_sum_f32<NV, G / 2>(buf, l);
+        else transpose_sum<NV, G / 2>(buf, l);
         tot[b] += buf[0];
       }
     });
