@@ -12,7 +12,9 @@
 // registers and form every column ma of the output row from them, with the
 // Clebsch-Gordan factors of the column sums as compile-time literals.  U of
 // the batch sits in LDS as [jju][32 atoms] (conflict-free 8-byte reads).
-// Z, Y and the energy accumulate in FP64 registers; Z is never stored.
+// Each Z entry is summed in FP32 over its rows (the column sums and at most
+// five row terms per half), then widened once: Y and the energy accumulate in
+// FP64 registers.  Z is never stored in the adjoint mode.
 //
 // The U and Y buffers use the atom-blocked layout [batch][jju][32][re, im]
 // written by the FP32 U kernel and read by the FP32 force kernel.
@@ -194,10 +196,12 @@ __device__ __forceinline__ void triple(const float2* __restrict__ UL, const floa
   constexpr int mb2hi = m2max(j1, j2, j, mb);
   constexpr int nb = mcount(j1, j2, j, mb);
   constexpr BlockPos bp = block_pos(J, j1, j2, j);
-  double zr[j + 1], zi[j + 1];
+  // Z of this half's mb1 rows: the FP32 column sums times the row factor,
+  // summed over at most five rows in FP32 and widened to FP64 once
+  float zrf[j + 1], zif[j + 1];
   static_for<0, j + 1>([&](auto mc) {
-    zr[decltype(mc)::value] = 0.0;
-    zi[decltype(mc)::value] = 0.0;
+    zrf[decltype(mc)::value] = 0.f;
+    zif[decltype(mc)::value] = 0.f;
   });
   for (int ib = h; ib < nb; ib += 2) {
     const int mb1 = mb1lo + ib, mb2 = mb2hi - ib;
@@ -206,7 +210,7 @@ __device__ __forceinline__ void triple(const float2* __restrict__ UL, const floa
     float2 u1[j1 + 1], u2[j2 + 1];
     static_for<0, j1 + 1>([&](auto k) { u1[decltype(k)::value] = r1[decltype(k)::value * AB]; });
     static_for<0, j2 + 1>([&](auto k) { u2[decltype(k)::value] = r2[decltype(k)::value * AB]; });
-    const double cb = (double)cgL[bp.cgoff + mb1 * (j2 + 1) + mb2];
+    const float cb = cgL[bp.cgoff + mb1 * (j2 + 1) + mb2];
     static_for<0, j + 1>([&](auto mc) {
       constexpr int ma = decltype(mc)::value;
       constexpr int lo = m1min(j1, j2, j, ma), hi2 = m2max(j1, j2, j, ma), na = mcount(j1, j2, j, ma);
@@ -218,15 +222,16 @@ __device__ __forceinline__ void triple(const float2* __restrict__ UL, const floa
         sr += c * (p.x * q.x - p.y * q.y);
         si += c * (p.x * q.y + p.y * q.x);
       });
-      zr[ma] += cb * (double)sr;
-      zi[ma] += cb * (double)si;
+      zrf[ma] += cb * sr;
+      zif[ma] += cb * si;
     });
   }
+  double zr[j + 1], zi[j + 1];
   if constexpr (DESC) {
     static_for<0, j + 1>([&](auto mc) {
       constexpr int ma = decltype(mc)::value;
-      zr[ma] += __shfl_xor(zr[ma], 32, 64);
-      zi[ma] += __shfl_xor(zi[ma], 32, 64);
+      zr[ma] = (double)(zrf[ma] + __shfl_xor(zrf[ma], 32, 64));
+      zi[ma] = (double)(zif[ma] + __shfl_xor(zif[ma], 32, 64));
     });
     if (store) {
       float2* zrow = Zat + bp.zoff + mb * (j + 1);
@@ -256,6 +261,10 @@ __device__ __forceinline__ void triple(const float2* __restrict__ UL, const floa
     }
     return;
   }
+  static_for<0, j + 1>([&](auto mc) {
+    zr[decltype(mc)::value] = (double)zrf[decltype(mc)::value];
+    zi[decltype(mc)::value] = (double)zif[decltype(mc)::value];
+  });
   if constexpr (j >= j1) {
     // this half's share of B_(j1,j2,j) = 2 sum_half Re(conj(U_j) Z) (the
     // middle row of an even level counts its first half and half its centre)
