@@ -830,7 +830,7 @@ SnapListDev build_snap_list(ipg_ctx* ctx, const DevSystem& s, const int32_t* d_a
     IPG_REQUIRE(!out.rev || flag, "internal: reverse-pair search without a flag");
     if (n > 0 && L.npairs > 0 && L.ellK > 0) {
       const int K = L.ellK;
-      const int LP = K >= 64 ? 64 : 32;
+      const int LP = K >= 128 ? 64 : 32;
       const size_t sm = sizeof(int32_t) * (size_t)(256 / LP) * K;
       IPG_REQUIRE(sm <= 64 * 1024, "internal: neighbor row buffer too wide");
       launch(ctx, "snap_list_finish", [&] {
@@ -1015,7 +1015,7 @@ SnapListDev build_snap_list_batch(ipg_ctx* ctx, const BatchSys& b, const SnapCut
   const int32_t* ell = scratch_t<int32_t>(ctx, (t + "_ell").c_str(), (size_t)std::max<int64_t>(n, 1) * K);
   out.rev = (need & SNAPL_SYM) ? scratch_t<int32_t>(ctx, (t + "_rev").c_str(), npa) : nullptr;
   if (n > 0 && np > 0) {
-    const int FL = K >= 64 ? 64 : 32;
+    const int FL = K >= 128 ? 64 : 32;
     const size_t sm = sizeof(int32_t) * (size_t)(256 / FL) * K;
     IPG_REQUIRE(sm <= 64 * 1024, "internal: neighbor row buffer too wide");
     IpgBox none{};
