@@ -204,15 +204,29 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_single(const int32_t* __res
   __shared__ int64_t pass_total;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   int64_t carry = 0;
+  // 16-byte loads and stores where the buffers allow (they are context
+  // scratch, so in practice always)
+  const bool vec = ((reinterpret_cast<uintptr_t>(c) | reinterpret_cast<uintptr_t>(off)) & 15) == 0;
   for (int64_t base = 0; base < n; base += (int64_t)SCAN_BLOCK * SCAN_PER) {
     const int64_t i0 = base + (int64_t)threadIdx.x * SCAN_PER;
     int32_t v[SCAN_PER];
     int64_t sum = 0;
+    if (vec && i0 + SCAN_PER <= n) {
+      const int4* c4 = reinterpret_cast<const int4*>(c + i0);
 #pragma unroll
-    for (int k = 0; k < SCAN_PER; k++) {
-      v[k] = i0 + k < n ? c[i0 + k] : 0;
-      sum += v[k];
+      for (int k = 0; k < SCAN_PER / 4; k++) {
+        const int4 q = c4[k];
+        v[4 * k] = q.x;
+        v[4 * k + 1] = q.y;
+        v[4 * k + 2] = q.z;
+        v[4 * k + 3] = q.w;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < SCAN_PER; k++) v[k] = i0 + k < n ? c[i0 + k] : 0;
     }
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; k++) sum += v[k];
     int64_t x = sum;  // inclusive wave scan of the thread sums
     for (int o = 1; o < 64; o <<= 1) {
       const int64_t y = __shfl_up(x, o, 64);
@@ -231,10 +245,20 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_single(const int32_t* __res
     }
     __syncthreads();
     int64_t e = carry + ws[wv] + x - sum;
+    if (vec && i0 + SCAN_PER <= n) {
+      longlong2* o2 = reinterpret_cast<longlong2*>(off + i0);
 #pragma unroll
-    for (int k = 0; k < SCAN_PER; k++) {
-      if (i0 + k < n) off[i0 + k] = cap > 0 && e > cap ? cap : e;
-      e += v[k];
+      for (int k = 0; k < SCAN_PER / 2; k++) {
+        const int64_t e0 = e, e1 = e + v[2 * k];
+        e = e1 + v[2 * k + 1];
+        o2[k] = make_longlong2(cap > 0 && e0 > cap ? cap : e0, cap > 0 && e1 > cap ? cap : e1);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < SCAN_PER; k++) {
+        if (i0 + k < n) off[i0 + k] = cap > 0 && e > cap ? cap : e;
+        e += v[k];
+      }
     }
     carry += pass_total;
     __syncthreads();  // ws and pass_total are rewritten by the next pass
