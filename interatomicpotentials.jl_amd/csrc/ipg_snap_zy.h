@@ -26,6 +26,19 @@ namespace zy {
 
 constexpr int AB = 32;  // atoms per batch (one per lane of a half wave)
 
+// x + (x of the lane 32 apart): the sum of the two halves' partial values,
+// by v_permlane32_swap (one lane sees its own value in one result and the
+// partner's in the other) instead of a shuffle through the LDS crossbar.
+__device__ __forceinline__ float half_sum(float x) { return swap_sum<32>(x, x); }
+__device__ __forceinline__ double half_sum(double x) {
+  const uint64_t b = (uint64_t)__double_as_longlong(x);
+  const auto lo = __builtin_amdgcn_permlane32_swap((uint32_t)b, (uint32_t)b, false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((uint32_t)(b >> 32), (uint32_t)(b >> 32), false, false);
+  const double d0 = __longlong_as_double((long long)(((uint64_t)hi[0] << 32) | lo[0]));
+  const double d1 = __longlong_as_double((long long)(((uint64_t)hi[1] << 32) | lo[1]));
+  return d0 + d1;
+}
+
 constexpr int cmin(int a, int b) { return a < b ? a : b; }
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
 constexpr double fact(int n) {
@@ -230,8 +243,8 @@ __device__ __forceinline__ void triple(const float2* __restrict__ UL, const floa
   if constexpr (DESC) {
     static_for<0, j + 1>([&](auto mc) {
       constexpr int ma = decltype(mc)::value;
-      zr[ma] = (double)(zrf[ma] + __shfl_xor(zrf[ma], 32, 64));
-      zi[ma] = (double)(zif[ma] + __shfl_xor(zif[ma], 32, 64));
+      zr[ma] = (double)half_sum(zrf[ma]);
+      zi[ma] = (double)half_sum(zif[ma]);
     });
     if (store) {
       float2* zrow = Zat + bp.zoff + mb * (j + 1);
@@ -316,8 +329,8 @@ __device__ __forceinline__ void unit(const float2* __restrict__ UL, const float*
   // the two halves hold partial sums over alternate mb1 rows
   static_for<0, j + 1>([&](auto mc) {
     constexpr int ma = decltype(mc)::value;
-    yr[ma] += __shfl_xor(yr[ma], 32, 64);
-    yi[ma] += __shfl_xor(yi[ma], 32, 64);
+    yr[ma] = half_sum(yr[ma]);
+    yi[ma] = half_sum(yi[ma]);
   });
   if (!store) return;
   // half h stores the columns ma = 2k + h (one instruction per column pair)
@@ -401,7 +414,7 @@ __global__ __launch_bounds__(64 * NW) void k_snap_zy32(SnapDev P, int64_t n, con
     }
     return;
   }
-  e += __shfl_xor(e, 32, 64);
+  e = zy::half_sum(e);
   if (h == 0) esum[w * zy::AB + at] = e;
   __syncthreads();
   if (w == 0 && h == 0 && valid) {
