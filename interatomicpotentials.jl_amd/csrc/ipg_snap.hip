@@ -386,6 +386,7 @@ __global__ void k_db_gather_sym(SnapDev P, SnapRun R, const T* __restrict__ dbli
   for (int t = threadIdx.x; t < nc3; t += blockDim.x) {
     double own = 0.0, wx = 0.0, wy = 0.0, wz = 0.0;
     if (Wa) {
+#pragma unroll 4
       for (int64_t q = q0; q < q1; q++) {
         const int64_t j = R.nbr[q];
         const double d = (double)dblist[q * nc3 + t];
