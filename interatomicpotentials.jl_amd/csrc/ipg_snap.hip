@@ -386,7 +386,7 @@ __global__ void k_db_gather_sym(SnapDev P, SnapRun R, const T* __restrict__ dbli
   for (int t = threadIdx.x; t < nc3; t += blockDim.x) {
     double own = 0.0, wx = 0.0, wy = 0.0, wz = 0.0;
     if (Wa) {
-#pragma unroll 4
+#pragma unroll 8
       for (int64_t q = q0; q < q1; q++) {
         const int64_t j = R.nbr[q];
         const double d = (double)dblist[q * nc3 + t];
@@ -408,14 +408,14 @@ __global__ void k_db_gather_sym(SnapDev P, SnapRun R, const T* __restrict__ dbli
         w[2] = wz;
       }
     } else {
-#pragma unroll 4
+#pragma unroll 8
       for (int64_t q = q0; q < q1; q++) own += (double)dblist[q * nc3 + t];
     }
     if (!dB) continue;
     double acc[8];
 #pragma unroll
     for (int s = 0; s < 8; s++) acc[s] = s == sa ? own : 0.0;
-#pragma unroll 4
+#pragma unroll 8
     for (int64_t q = q0; q < q1; q++) {
       const int sc = R.species ? R.species[R.nbr[q]] : 0;
       const double v = (double)dblist[(int64_t)R.rev[q] * nc3 + t];
