@@ -159,3 +159,23 @@ def test_full_size_mixed_properties(ipg):
     e32, f32 = ipg.energy_and_force(s, pot, precision="fp32")
     assert abs(e32 - e) / abs(e) < 1e-5
     assert np.abs(f32 - f).max() / np.abs(f).max() < 1e-4
+
+
+def test_dense_clump_overflows_neighbor_rows(ipg, orc):
+    """A dense clump in a dilute periodic box: the clump's rows outgrow the
+    row width estimated from the mean density, so the call falls back to
+    the counted CSR list; energy, forces and virial still match the
+    oracle."""
+    rng = np.random.default_rng(21)
+    L = 60.0
+    dilute = rng.uniform(0, L, size=(600, 3))
+    clump = rng.uniform(25.0, 35.0, size=(300, 3))  # ~270 neighbors each, mean density gives ~4
+    X = np.concatenate([dilute, clump])
+    s = ipg.AtomsSystem(X, [L, L, L], (True, True, True), ["Ar"] * len(X), unit="angstrom")
+    p = ipg.LennardJones(0.0104 * ipg.eV, 2.0 * ipg.angstrom, 6.0 * ipg.angstrom, ["Ar"])
+    e, f = ipg.energy_and_force(s, p)
+    vir = ipg.virial_stress(s, p)
+    eo, fo, vo = oracle_eval(orc, s, p)
+    close(e, eo)
+    close(f, fo)
+    close(vir, vo)

@@ -153,3 +153,28 @@ def test_adjoint_fp32_every_level(ipg, orc, J, bzero):
     e32, f32 = ipg.snap.energy_and_force(s, lbp, path="adjoint_fp32")
     assert abs(e32 - ref["energy"]) <= 1e-5 * max(abs(ref["energy"]), 1.0)
     assert rel(f32, ref["forces"]) < 1e-4
+
+
+def test_cutoff_list_row_overflow_falls_back(ipg, orc):
+    """A dense clump in a dilute box overflows the CUTOFF list's row buffer
+    (sized from the mean density); the counted list that replaces it gives
+    the oracle's descriptors and forces."""
+    from ipgpu.system import AtomsSystem
+
+    rng = np.random.default_rng(22)
+    L = 40.0
+    X = np.concatenate([rng.uniform(0, L, size=(300, 3)), rng.uniform(18.0, 24.0, size=(150, 3))])
+    names = ["W", "Mo"]
+    syms = [names[k] for k in rng.integers(0, 2, size=len(X))]
+    s = AtomsSystem(X, [L, L, L], (True, True, True), syms, unit="angstrom")
+    g, o, sp, (off, nbr, r) = make(ipg, orc, s, 4)
+    assert np.diff(off).max() > 100  # rows far wider than the mean density suggests
+    B, dB, _ = ipg.snap.compute_all_descriptors(s, g, want=("B", "dB"))
+    Bo, dBo, _ = o.all_descriptors(s.positions, sp, off, nbr, r)
+    assert rel(B, Bo) < 1e-10
+    assert rel(dB, dBo.transpose(0, 2, 1)) < 1e-10
+    beta = np.random.default_rng(5).normal(size=2 * g.ncoeff)
+    lbp = ipg.snap.LBasisPotential(beta, 0.0, g)
+    e, f = ipg.snap.energy_and_force(s, lbp, path="adjoint")
+    e0, f0 = ipg.snap.energy_and_force(s, lbp, path="plain")
+    assert abs(e - e0) <= 1e-10 * abs(e0) and rel(f, f0) < 1e-10
