@@ -492,6 +492,17 @@ def main():
     t64a = time_steps(lambda: snap_step(sn, dsys, d, IPG_SNAP_ADJOINT_FP64), steps64, 2)
     f64a = d["f"].download()
     log("fp64 plain: %.3f ms/step, fp64 adjoint %.3f ms/step" % (t64 * 1e3, t64a * 1e3))
+    # Backward of a force-matching fit: the pullback of F(x, beta) for a
+    # seeded gF, through the host-buffer entry point (positions and gF
+    # uploaded, both gradients downloaded inside the time)
+    lbp = ipgpu.snap.LBasisPotential(beta, 0.0, sn)
+    gF = np.random.default_rng(57).normal(size=(n, 3))
+    backward = {"entry": "ipg_snap_force_vjp (host buffers)"}
+    for prec in ("fp64", "fp32"):
+        tb = time_steps(lambda: ipgpu.snap.force_vjp(s, lbp, gF, precision=prec), steps64, 1)
+        backward[prec + "_ms"] = tb * 1e3
+        backward[prec + "_atoms_per_s"] = n / tb
+    log("force pullback: fp64 %.3f ms, fp32 %.3f ms" % (backward["fp64_ms"], backward["fp32_ms"]))
 
     # FP32 vs FP64 force errors (SNAP energies/forces are in the units of beta.B)
     df = f32 - f64
@@ -529,6 +540,7 @@ def main():
             "timed_region": "graph replay, HIP events around %s only" % dom,
             "graph": ginfo,
             "fp32_vs_fp64_error": err,
+            "force_matching_backward": backward,
         },
     }
     if not a.no_lj:

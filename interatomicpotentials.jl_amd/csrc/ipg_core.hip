@@ -93,6 +93,16 @@ __global__ void k_fill_zero(uint32_t* __restrict__ p, int64_t nwords) {
   for (; i < nwords; i += stride) p[i] = 0u;
 }
 
+// Two ranges in one launch: words [0, n) of p, then [0, m) of q.
+__global__ void k_fill_zero2(uint32_t* __restrict__ p, int64_t n, uint32_t* __restrict__ q, int64_t m) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n + m; i += stride) {
+    if (i < n) p[i] = 0u;
+    else q[i - n] = 0u;
+  }
+}
+
 // Zero-fill with a kernel (no hipMemsetAsync, so the call stays capturable
 // in a HIP graph).  bytes must be a multiple of 4.
 void zero(ipg_ctx* ctx, void* dst, size_t bytes) {
@@ -101,6 +111,18 @@ void zero(ipg_ctx* ctx, void* dst, size_t bytes) {
   unsigned g = grid_for(nw, 256);
   if (g > 4096) g = 4096;
   launch(ctx, "fill_zero", [&] { k_fill_zero<<<g, 256, 0, ctx->stream>>>((uint32_t*)dst, nw); });
+}
+
+// Two zero-fills in one launch (one kernel instead of two in a captured
+// step).  Byte counts must be multiples of 4; a zero count skips that range.
+void zero2(ipg_ctx* ctx, void* dst, size_t bytes, void* dst2, size_t bytes2) {
+  if (bytes2 == 0) return zero(ctx, dst, bytes);
+  if (bytes == 0) return zero(ctx, dst2, bytes2);
+  const int64_t n = (int64_t)(bytes / 4), m = (int64_t)(bytes2 / 4);
+  unsigned g = grid_for(n + m, 256);
+  if (g > 4096) g = 4096;
+  launch(ctx, "fill_zero",
+         [&] { k_fill_zero2<<<g, 256, 0, ctx->stream>>>((uint32_t*)dst, n, (uint32_t*)dst2, m); });
 }
 
 // ---------------------------------------------------------------------------
@@ -388,8 +410,51 @@ __global__ __launch_bounds__(RED1_BLOCK) void k_reduce_rows_narrow(const double*
   }
 }
 
+// Wide rows (width > 8): a lane per column, so each wave loads 64
+// consecutive doubles of one row.  Blocks take fixed chunks of rows (the 4
+// waves of a block every 4th row of the chunk, combined in wave order); the
+// second kernel adds the chunk partials in chunk order.
+constexpr int REDW_CHUNK = 256;
+__global__ __launch_bounds__(256) void k_reduce_wide_p1(const double* __restrict__ rows, int64_t n, int width,
+                                                        int64_t chunk, double* __restrict__ part) {
+  __shared__ double sh[4][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int col = blockIdx.x * 64 + lane;
+  const int64_t lo = (int64_t)blockIdx.y * chunk;
+  const int64_t hi = lo + chunk < n ? lo + chunk : n;
+  double acc = 0.0;
+  if (col < width)
+    for (int64_t i = lo + wv; i < hi; i += 4) acc += rows[i * width + col];
+  sh[wv][lane] = acc;
+  __syncthreads();
+  if (wv == 0 && col < width)
+    part[(int64_t)blockIdx.y * width + col] = ((sh[0][lane] + sh[1][lane]) + sh[2][lane]) + sh[3][lane];
+}
+
+__global__ void k_reduce_wide_p2(const double* __restrict__ part, int64_t nchunk, int width, double* __restrict__ out,
+                                 int accumulate, double offset) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= width) return;
+  double t = 0.0;
+  for (int64_t c = 0; c < nchunk; c++) t += part[c * width + col];
+  out[col] = offset + (accumulate ? out[col] + t : t);
+}
+
 void reduce_rows(ipg_ctx* ctx, const double* d_rows, int64_t n, int width, double* d_out, bool accumulate,
                  double offset) {
+  if (width > 8 && n > 0) {
+    int64_t chunk = REDW_CHUNK;
+    if ((n + chunk - 1) / chunk > 65535) chunk = (n + 65534) / 65535;
+    const int64_t nchunk = (n + chunk - 1) / chunk;
+    double* part = scratch_t<double>(ctx, "reduce_wide_part", nchunk * (int64_t)width);
+    launch(ctx, "reduce_rows", [&] {
+      k_reduce_wide_p1<<<dim3((unsigned)((width + 63) / 64), (unsigned)nchunk), 256, 0, ctx->stream>>>(
+          d_rows, n, width, chunk, part);
+      k_reduce_wide_p2<<<grid_for(width, 256), 256, 0, ctx->stream>>>(part, nchunk, width, d_out,
+                                                                       accumulate ? 1 : 0, offset);
+    });
+    return;
+  }
   if (n <= 64 * RED1_BLOCK && width >= 1 && width <= 8) {
     launch(ctx, "reduce_rows", [&] {
       const int acc = accumulate ? 1 : 0;

@@ -143,6 +143,7 @@ void h2d(ipg_ctx* ctx, void* dst, const void* src, size_t bytes);
 void d2h(ipg_ctx* ctx, void* dst, const void* src, size_t bytes);
 void sync(ipg_ctx* ctx);
 void zero(ipg_ctx* ctx, void* dst, size_t bytes);
+void zero2(ipg_ctx* ctx, void* dst, size_t bytes, void* dst2, size_t bytes2);
 
 // Add the elapsed times of event pairs recorded by a graph replay to the
 // statistics (the events stay owned by the graph).

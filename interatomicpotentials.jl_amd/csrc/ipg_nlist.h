@@ -73,10 +73,12 @@ struct SnapCut {
 // gets bit 1 (stage A / CUTOFF list) or 2 (stage B) when the true count was
 // larger; bit 4 marks a missing reverse pair.  CUTOFF mode's first pass
 // also writes rows of width ellK (0: estimated from the density) and builds
-// the sorted list from them; bit 8 marks a row wider than ellK.
+// the sorted list from them; bit 8 marks a row wider than ellK.  With
+// clear_flag, CUTOFF mode zeroes *flag in the list's first launch (the
+// caller's own clear is then not needed); REFERENCE mode ignores it.
 SnapListDev build_snap_list(ipg_ctx* ctx, const DevSystem& s, const int32_t* d_appear, const SnapCut& cut, int need,
                             const char* tag, int64_t capA = 0, int64_t capB = 0, int32_t* flag = nullptr,
-                            int ellK = 0);
+                            int ellK = 0, bool clear_flag = false);
 
 // Packed configurations on the device: atoms of configuration c are
 // start[c] .. start[c+1]-1, cfg[i] is the configuration of atom i, boxes[c]

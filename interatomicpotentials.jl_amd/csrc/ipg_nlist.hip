@@ -441,13 +441,15 @@ DevSystem upload_system(ipg_ctx* ctx, const ipg_system* s, const char* tag) {
 // bit 8 of *capflag; exact sizing then runs the fill and sort passes.
 template <typename Alloc>
 static DevList build_list_impl(ipg_ctx* ctx, const DevSystem& s, double rcut, int kind, bool sorted, bool want_disp,
-                               int64_t cap, int32_t* capflag, const SnapCutDev* sc, Alloc&& alloc, int ellK = 0) {
+                               int64_t cap, int32_t* capflag, const SnapCutDev* sc, Alloc&& alloc, int ellK = 0,
+                               bool clear_flag = false) {
   IPG_REQUIRE(std::isfinite(rcut) && rcut > 0, "rcutoff must be finite and > 0");
+  IPG_REQUIRE(!clear_flag || capflag, "internal: flag clear without a flag");
   DevList L;
   L.n = s.n;
   L.off = (int64_t*)alloc("off", sizeof(int64_t) * (s.n + 1));
   if (s.n == 0) {
-    zero(ctx, L.off, sizeof(int64_t));
+    zero2(ctx, L.off, sizeof(int64_t), capflag, clear_flag ? sizeof(int32_t) : 0);
     L.nbr = (int32_t*)alloc("nbr", 4);
     L.disp = want_disp ? (double*)alloc("disp", 8) : nullptr;
     return L;
@@ -475,7 +477,9 @@ static DevList build_list_impl(ipg_ctx* ctx, const DevSystem& s, double rcut, in
   double4* spos = scratch_t<double4>(ctx, "nl_spos", s.n);
   float4* sposf = scratch_t<float4>(ctx, "nl_sposf", s.n);
   int32_t* cnt = scratch_t<int32_t>(ctx, "nl_cnt", s.n);
-  zero(ctx, ccount, sizeof(int32_t) * (2 * g.ncell + 2));
+  // the cell counters, and the caller's flag word when this build starts
+  // its step (one fill launch for both)
+  zero2(ctx, ccount, sizeof(int32_t) * (2 * g.ncell + 2), capflag, clear_flag ? sizeof(int32_t) : 0);
   const unsigned gb = grid_for(s.n, 256);
   launch(ctx, "cell_assign", [&] { k_cell_assign<<<gb, 256, 0, ctx->stream>>>(s.pos, s.n, g, cellof, ccount); });
   exclusive_scan(ctx, ccount, g.ncell, cstart, false);
@@ -808,7 +812,8 @@ __global__ void k_rev_search(int64_t np_max, int64_t n, const int64_t* __restric
 }
 
 SnapListDev build_snap_list(ipg_ctx* ctx, const DevSystem& s, const int32_t* d_appear, const SnapCut& cut, int need,
-                            const char* tag, int64_t capA, int64_t capB, int32_t* flag, int ellK) {
+                            const char* tag, int64_t capA, int64_t capB, int32_t* flag, int ellK,
+                            bool clear_flag) {
   std::string t(tag);
   SnapListDev out;
   const int64_t n = s.n;
@@ -822,7 +827,7 @@ SnapListDev build_snap_list(ipg_ctx* ctx, const DevSystem& s, const int32_t* d_a
     DevList L = build_list_impl(
         ctx, s, cut.query, KIND_FULL, true, false, capB, flag, &cd,
         [&](const char* what, size_t bytes) { return scratch(ctx, (t + "_" + what).c_str(), bytes); },
-        ellK != 0 ? ellK : -1);
+        ellK != 0 ? ellK : -1, clear_flag);
     out.list = L;
     out.npairs_a = L.npairs;
     out.ellK = L.ellK_req;

@@ -507,8 +507,9 @@ static SnapCall snap_prepare(ipg_snap* sn, const DevSystem& s, int mode, const i
   IPG_REQUIRE(mode == IPG_SNAP_LIST_REFERENCE || mode == IPG_SNAP_LIST_CUTOFF, "unknown SNAP list mode");
   SnapCall c;
   c.flag = scratch_t<int32_t>(sn->ctx, "snap_flag", 1);
-  zero(sn->ctx, c.flag, sizeof(int32_t));
   SnapCut cut = snap_make_cut(sn->host, mode);
+  // CUTOFF mode: the list build clears the flag in its first fill launch
+  if (cut.reference) zero(sn->ctx, c.flag, sizeof(int32_t));
   const int32_t* ap = d_appear ? d_appear : s.species;
   if (cut.reference && !ap) {
     // all atoms are species 0: element index 0 everywhere
@@ -516,7 +517,7 @@ static SnapCall snap_prepare(ipg_snap* sn, const DevSystem& s, int mode, const i
     zero(sn->ctx, z, sizeof(int32_t) * (s.n > 0 ? s.n : 1));
     ap = z;
   }
-  c.L = build_snap_list(sn->ctx, s, ap, cut, need, "snap_list", capA, capB, c.flag, ellK);
+  c.L = build_snap_list(sn->ctx, s, ap, cut, need, "snap_list", capA, capB, c.flag, ellK, !cut.reference);
   c.R.n = s.n;
   c.R.npairs = c.L.list.npairs;
   c.R.d_np = c.L.list.exact ? nullptr : c.L.list.off + s.n;
@@ -818,7 +819,7 @@ static void snap_energy_forces_call(ipg_snap* sn, const DevSystem& s, int mode, 
   g.capB = nb + nb / 8 + 256;
   g.avg_nbr = (double)nb / (double)s.n;
   g.ellK = c.L.ellK;
-  g.flag = c.flag;  // snap_prepare clears it at the start of every step
+  g.flag = c.flag;  // cleared by the first launch of every step
   auto step = [&] {
     SnapCall cc = snap_prepare(sn, s, mode, d_appear, ef_need(path), g.capA, g.capB, g.ellK);
     cc.R.avg_nbr = g.avg_nbr;

@@ -6,6 +6,8 @@ finishes in seconds.
   FP32 adjoint forces within 1e-5 RMS of the FP64 plain forces (in the
   units of beta . B), FP64 adjoint within 1e-10 relative of FP64 plain,
   total force zero (translation invariance) to 1e-9 relative.
+- The same workload as a force-matching fit: the pullback of F against
+  central differences of the loss in beta and in positions.
 - SNAP 2J=6, 2000-atom bcc crystal: B and dB against the oracle (1e-10
   relative) in CUTOFF mode; REFERENCE mode finds no neighbors there, so B is
   the self term alone and dB is zero.
@@ -46,6 +48,52 @@ def test_snap_2J8_8192_atoms_paths(ipg):
     assert abs(e32 - e64) <= 1e-6 * len(s) * max(1.0, abs(e64) / len(s))
     for f in (f64, fa):
         assert np.abs(f.sum(0)).max() <= 1e-9 * np.abs(f).sum()
+
+
+def test_snap_2J8_8192_atoms_force_matching_gradient(ipg):
+    """Force-matching loss L = mean |F - F_target|^2 over the 8192-atom
+    workload, F_target from a second seeded beta.  The pullback of F (FP64)
+    at gF = dL/dF must equal central differences of L: in beta (L is
+    quadratic in beta, so the difference is exact up to rounding) and in a
+    seeded subset of coordinates (h = 1e-4 A).  The FP32 pullback stays
+    within 1e-5 of the FP64 one, relative to the largest entry."""
+    from ipgpu import workloads
+
+    s = workloads.snap_bcc(ncell=16, a=3.16, jitter=0.05, nspecies=2, seed=5)
+    names = ["W", "Mo"]
+    sn = ipg.snap.SNAP(len(s), 8, names, 5.0, 0.0, 0.99363, [0.5, 0.5], [1.0, 0.9], switch_flag=True,
+                       list_mode="cutoff")
+    rng = np.random.default_rng(56)
+    beta = rng.normal(scale=0.1, size=2 * sn.ncoeff)
+    beta_t = rng.normal(scale=0.1, size=2 * sn.ncoeff)
+    F_t = ipg.snap.energy_and_force(s, ipg.snap.LBasisPotential(beta_t, 0.0, sn), path="adjoint")[1]
+
+    def loss(X=None, b=beta):
+        sx = s if X is None else type(s)(X, s.box, s.periodic, s.symbols, unit="angstrom")
+        F = ipg.snap.energy_and_force(sx, ipg.snap.LBasisPotential(b, 0.0, sn), path="adjoint")[1]
+        return float(np.mean((F - F_t) ** 2)), F
+
+    L0, F = loss()
+    gF = 2.0 * (F - F_t) / F.size
+    lbp = ipg.snap.LBasisPotential(beta, 0.0, sn)
+    gb, gp = ipg.snap.force_vjp(s, lbp, gF, precision="fp64")
+    for c in rng.choice(beta.size, 6, replace=False):
+        hb = 1e-3
+        bp, bm = beta.copy(), beta.copy()
+        bp[c] += hb
+        bm[c] -= hb
+        fd = (loss(b=bp)[0] - loss(b=bm)[0]) / (2 * hb)
+        assert abs(gb[c] - fd) <= 1e-8 * np.abs(gb).max(), (c, gb[c], fd)
+    h = 1e-4
+    for a, k in zip(rng.choice(len(s), 6, replace=False), rng.integers(0, 3, 6)):
+        Xp, Xm = s.positions.copy(), s.positions.copy()
+        Xp[a, k] += h
+        Xm[a, k] -= h
+        fd = (loss(Xp)[0] - loss(Xm)[0]) / (2 * h)
+        assert abs(gp[a, k] - fd) <= 1e-5 * np.abs(gp).max(), (a, k, gp[a, k], fd)
+    gb32, gp32 = ipg.snap.force_vjp(s, lbp, gF, precision="fp32")
+    print("FP32 pullback vs FP64: beta %.3g, positions %.3g" % (rel(gb32, gb), rel(gp32, gp)))
+    assert rel(gb32, gb) < 1e-5 and rel(gp32, gp) < 1e-5
 
 
 def test_snap_2J6_2000_atoms_vs_oracle(ipg, orc):
