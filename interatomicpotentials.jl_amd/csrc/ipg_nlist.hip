@@ -203,12 +203,16 @@ struct Prefilter {
 
 // LP lanes per atom scan the surrounding cells; each cell's candidates are
 // taken LP at a time.  A cheap FP32 minimum-image distance rejects far
-// candidates; the rest get the exact FP64 PeriodicEuclidean test.  Hits are
-// placed with a ballot, so the output order is fixed (cell order, then slot).
+// candidates; the survivors are queued in LDS in candidate order, and the
+// exact FP64 PeriodicEuclidean test runs over the queue LP entries at a time
+// (so the FP64 code runs on full lanes, not once per candidate chunk).  Hits
+// are placed with a ballot, so the output order is fixed (cell order, then
+// slot).
 // MODE: SCAN_COUNT (cnt[i]), SCAN_FILL (rows at off[i], writes stop at cap)
 // or SCAN_ELL (the first K hits of atom i in ell[i*K ...] and their count
 // in cnt[i]; more than K hits raise bit 8 of *flag).
 enum { SCAN_COUNT = 0, SCAN_FILL = 1, SCAN_ELL = 2 };
+constexpr int SCAN_QCAP = 4;  // queue capacity per group, in chunks of LP
 template <int LP, int MODE, bool SC>
 __global__ __launch_bounds__(256) void k_list_scan(int64_t n, Grid g, Prefilter pf, const int32_t* __restrict__ catom,
                                                    const double4* __restrict__ spos, const float4* __restrict__ sposf,
@@ -238,31 +242,21 @@ __global__ __launch_bounds__(256) void k_list_scan(int64_t n, Grid g, Prefilter 
 #pragma unroll
   for (int t = 0; t < NS; t++) cell_range(g, cell, cstart, l + t * LP, rs[t], re[t]);
   const int base = lane - l;
-  for (int r = 0; r < NR; r++) {
-    const int t = r / LP;
-    int32_t a0 = rs[0], a1 = re[0];
-#pragma unroll
-    for (int u = 1; u < NS; u++)
-      if (t == u) a0 = rs[u], a1 = re[u];
-    const int32_t s0 = __shfl(a0, base + r % LP, 64), s1 = __shfl(a1, base + r % LP, 64);
-    for (int64_t s = s0; s < s1; s += LP) {
-      const int64_t sl = s + l;
+  // survivors of the prefilter (cell-order slots), one queue per group
+  __shared__ int32_t queue[256 / LP][SCAN_QCAP * LP];
+  int32_t* qg = queue[threadIdx.x / LP];
+  int qn = 0;  // the same in every lane of the group
+  auto drain = [&] {
+    __builtin_amdgcn_wave_barrier();  // queue stores before the loads below
+    for (int t = 0; t < qn; t += LP) {
       bool hit = false;
       int32_t j = -1;
-      if (sl < s1) {
+      if (t + l < qn) {
+        const int32_t sl = qg[t + l];
         j = catom[sl];
-        const float4 qj = sposf[sl];
         const double4 pj = spos[sl];
-        if (keep_pair(kind, i, j)) {
-          float dx = qj.x - qi.x, dy = qj.y - qi.y, dz = qj.z - qi.z;
-          if (g.box.per[0]) dx -= pf.L[0] * rintf(dx * pf.invL[0]);
-          if (g.box.per[1]) dy -= pf.L[1] * rintf(dy * pf.invL[1]);
-          if (g.box.per[2]) dz -= pf.L[2] * rintf(dz * pf.invL[2]);
-          if (dx * dx + dy * dy + dz * dz <= pf.r2) {
-            hit = ipg_pe_sumsq(g.box, pi.x, pi.y, pi.z, pj.x, pj.y, pj.z) <= smax;
-            if constexpr (SC) hit = hit && snap_cut_ok(g.box, pi, pj, sc);
-          }
-        }
+        hit = ipg_pe_sumsq(g.box, pi.x, pi.y, pi.z, pj.x, pj.y, pj.z) <= smax;
+        if constexpr (SC) hit = hit && snap_cut_ok(g.box, pi, pj, sc);
       }
       const uint64_t m = __ballot(hit) & gmask;
       if constexpr (MODE == SCAN_FILL) {
@@ -276,7 +270,37 @@ __global__ __launch_bounds__(256) void k_list_scan(int64_t n, Grid g, Prefilter 
         count += __popcll(m);
       }
     }
+    __builtin_amdgcn_wave_barrier();  // loads done before the queue is refilled
+    qn = 0;
+  };
+  for (int r = 0; r < NR; r++) {
+    const int t = r / LP;
+    int32_t a0 = rs[0], a1 = re[0];
+#pragma unroll
+    for (int u = 1; u < NS; u++)
+      if (t == u) a0 = rs[u], a1 = re[u];
+    const int32_t s0 = __shfl(a0, base + r % LP, 64), s1 = __shfl(a1, base + r % LP, 64);
+    for (int64_t s = s0; s < s1; s += LP) {
+      const int64_t sl = s + l;
+      bool pass = false;
+      if (sl < s1) {
+        const int32_t j = catom[sl];
+        const float4 qj = sposf[sl];
+        if (keep_pair(kind, i, j)) {
+          float dx = qj.x - qi.x, dy = qj.y - qi.y, dz = qj.z - qi.z;
+          if (g.box.per[0]) dx -= pf.L[0] * rintf(dx * pf.invL[0]);
+          if (g.box.per[1]) dy -= pf.L[1] * rintf(dy * pf.invL[1]);
+          if (g.box.per[2]) dz -= pf.L[2] * rintf(dz * pf.invL[2]);
+          pass = dx * dx + dy * dy + dz * dz <= pf.r2;
+        }
+      }
+      const uint64_t m = __ballot(pass) & gmask;
+      if (pass) qg[qn + __popcll(m & below)] = (int32_t)sl;
+      qn += __popcll(m);
+      if (qn > (SCAN_QCAP - 1) * LP) drain();  // room for one more chunk
+    }
   }
+  if (qn > 0) drain();
   if (MODE != SCAN_FILL && l == 0) {
     if (MODE == SCAN_ELL && count > ellK) {
       atomicOr(flag, 8);
