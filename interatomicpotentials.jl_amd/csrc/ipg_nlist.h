@@ -25,6 +25,8 @@ struct DevList {
   int ellK_req = 0;        // row width the first pass used (0: no row buffer)
   const int32_t* ell = nullptr;
   const int32_t* cnt = nullptr;
+  bool deferred = false;   // rows returned before the overflow bit was read: npairs is an estimate, off is
+                           // unset, and the caller reads bit 8 of its flag after using the rows
 };
 
 DevSystem make_dev_system(const ipg_system* s);  // s holds device pointers
@@ -37,8 +39,11 @@ DevSystem upload_system(ipg_ctx* ctx, const ipg_system* s, const char* tag);  //
 // neighbors into rows of width ellK (-1: from the density) in the order the
 // CSR list would hold them; when no row overflowed the list comes back with
 // ellK > 0, the rows in ell and their lengths in cnt, and nbr unfilled.
+// defer (exact sizing with rows only): return the rows right after the
+// scan, with no offsets and no host synchronization (DevList::deferred).
 DevList build_list(ipg_ctx* ctx, const DevSystem& s, double rcut, int kind, bool sorted, bool want_disp,
-                   const char* tag, int64_t cap = 0, int32_t* capflag = nullptr, int ellK = 0);
+                   const char* tag, int64_t cap = 0, int32_t* capflag = nullptr, int ellK = 0,
+                   bool defer = false);
 
 // Same, into caller-owned device buffers allocated by the function (for
 // ipg_nlist objects); frees nothing.

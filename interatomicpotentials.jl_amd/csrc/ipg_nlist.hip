@@ -466,7 +466,7 @@ DevSystem upload_system(ipg_ctx* ctx, const ipg_system* s, const char* tag) {
 template <typename Alloc>
 static DevList build_list_impl(ipg_ctx* ctx, const DevSystem& s, double rcut, int kind, bool sorted, bool want_disp,
                                int64_t cap, int32_t* capflag, const SnapCutDev* sc, Alloc&& alloc, int ellK = 0,
-                               bool clear_flag = false) {
+                               bool clear_flag = false, bool defer = false) {
   IPG_REQUIRE(std::isfinite(rcut) && rcut > 0, "rcutoff must be finite and > 0");
   IPG_REQUIRE(!clear_flag || capflag, "internal: flag clear without a flag");
   DevList L;
@@ -517,13 +517,11 @@ static DevList build_list_impl(ipg_ctx* ctx, const DevSystem& s, double rcut, in
   const unsigned gl = grid_for(s.n * LP, 256);
   SnapCutDev scd{};
   if (sc) scd = *sc;
-  if (ellK < 0) {
-    // row width from the expected neighbor count at the grid's mean density
-    double vol = (double)g.ncell;
-    for (int d = 0; d < 3; d++) vol *= g.w[d];
-    const double expect = (double)s.n / vol * 4.18879020478639 * rcut * rcut * rcut;
-    ellK = (int)std::min<double>(4096.0, std::ceil((1.5 * expect + 32.0) / 32.0) * 32.0);
-  }
+  // expected neighbors per atom at the grid's mean density
+  double vol = (double)g.ncell;
+  for (int d = 0; d < 3; d++) vol *= g.w[d];
+  const double expect = std::min((double)s.n / vol * 4.18879020478639 * rcut * rcut * rcut, (double)s.n);
+  if (ellK < 0) ellK = (int)std::min<double>(4096.0, std::ceil((1.5 * expect + 32.0) / 32.0) * 32.0);
   const bool ell_mode = ellK > 0 && (sc ? sorted : !sorted);
   L.ellK_req = ell_mode ? ellK : 0;
   int32_t* ell = ell_mode ? scratch_t<int32_t>(ctx, "nl_ell", (size_t)s.n * ellK) : nullptr;
@@ -548,6 +546,15 @@ static DevList build_list_impl(ipg_ctx* ctx, const DevSystem& s, double rcut, in
     launch(ctx, "nlist_count", [&] { dispatch(std::integral_constant<int, SCAN_ELL>(), nullptr, ell); });
   } else {
     launch(ctx, "nlist_count", [&] { dispatch(std::integral_constant<int, SCAN_COUNT>(), nullptr, nullptr); });
+  }
+  if (ell_mode && defer && cap <= 0) {
+    L.npairs = (int64_t)(expect * (double)s.n);
+    L.exact = false;
+    L.ellK = ellK;
+    L.ell = ell;
+    L.cnt = cnt;
+    L.deferred = true;
+    return L;
   }
   // exact size (reads the total back) or a capacity given by the caller
   // (no host synchronization; the device total is L.off[n], clamped to the
@@ -603,12 +610,14 @@ static DevList build_list_impl(ipg_ctx* ctx, const DevSystem& s, double rcut, in
 }
 
 DevList build_list(ipg_ctx* ctx, const DevSystem& s, double rcut, int kind, bool sorted, bool want_disp,
-                   const char* tag, int64_t cap, int32_t* capflag, int ellK) {
+                   const char* tag, int64_t cap, int32_t* capflag, int ellK, bool defer) {
   std::string t(tag);
   IPG_REQUIRE(ellK == 0 || (!sorted && capflag), "internal: row-buffer lists are unsorted and need a flag");
+  IPG_REQUIRE(!defer || (ellK != 0 && !want_disp), "internal: a deferred list has rows only");
   return build_list_impl(
       ctx, s, rcut, kind, sorted, want_disp, cap, capflag, nullptr,
-      [&](const char* what, size_t bytes) { return scratch(ctx, (t + "_" + what).c_str(), bytes); }, ellK);
+      [&](const char* what, size_t bytes) { return scratch(ctx, (t + "_" + what).c_str(), bytes); }, ellK, false,
+      defer);
 }
 
 DevList build_list_owned(ipg_ctx* ctx, const DevSystem& s, double rcut, int kind, bool sorted, bool want_disp) {

@@ -315,11 +315,27 @@ static void pair_dev(ipg_ctx* ctx, const DevSystem& s, const ipg_pair_term* term
     L = *prebuilt;
     IPG_REQUIRE(L.n == s.n, "list natoms does not match the system");
   } else {
-    // one scan into neighbor rows (no second, filling scan) when no row
-    // outgrows the width estimated from the density
+    // one scan into neighbor rows of the width estimated from the density,
+    // used at once; the overflow bit is read after the pair kernel (one
+    // synchronization, and none in the middle of the call)
     int32_t* flag = scratch_t<int32_t>(ctx, "pair_list_flag", 1);
     zero(ctx, flag, sizeof(int32_t));
-    L = build_list(ctx, s, rc, KIND_FULL, false, false, "pair_list", 0, flag, -1);
+    L = build_list(ctx, s, rc, KIND_FULL, false, false, "pair_list", 0, flag, -1, true);
+    if (L.deferred) {
+      PairTable tab = prepare_terms(terms, nterms, list_rcut);
+      double* ev = scratch_t<double>(ctx, "pair_ev", 8);
+      double* f = d_forces ? d_forces : scratch_t<double>(ctx, "pair_fscratch", 3 * (s.n > 0 ? s.n : 1));
+      run_pair(ctx, s, L, tab, precision, f, ev);
+      int32_t over = 0;
+      d2h(ctx, &over, flag, sizeof(int32_t));
+      sync(ctx);
+      if ((over & 8) == 0) {
+        launch(ctx, "split_ev", [&] { k_split_ev<<<1, 64, 0, ctx->stream>>>(ev, d_energy, d_virial); });
+        return;
+      }
+      // a row outgrew the width: a counted list, and the pair kernel again
+      L = build_list(ctx, s, rc, KIND_FULL, false, false, "pair_list", 0, flag, 0);
+    }
   }
   // terms whose rcut is below the list's test their own membership
   PairTable tab = prepare_terms(terms, nterms, list_rcut);
