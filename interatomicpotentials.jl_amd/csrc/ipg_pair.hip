@@ -304,9 +304,14 @@ static void run_pair(ipg_ctx* ctx, const DevSystem& s, const DevList& L, const P
   reduce_rows(ctx, part, nblk, 7, d_ev, false);
 }
 
-static void pair_dev(ipg_ctx* ctx, const DevSystem& s, const ipg_pair_term* terms, int nterms, int precision,
-                     double* d_energy, double* d_forces, double* d_virial, const DevList* prebuilt,
-                     double prebuilt_rcut) {
+// Without a prebuilt list the call scans into neighbor rows of the width
+// estimated from the density and uses them at once; bit 8 of the returned
+// flag word (null when there is nothing to check) says a row outgrew that
+// width, and the call must then be repeated with counted = true.  The flag
+// is read by the caller, together with its results (one synchronization).
+static int32_t* pair_dev(ipg_ctx* ctx, const DevSystem& s, const ipg_pair_term* terms, int nterms, int precision,
+                         double* d_energy, double* d_forces, double* d_virial, const DevList* prebuilt,
+                         double prebuilt_rcut, bool counted = false) {
   IPG_REQUIRE(precision == IPG_FP64 || precision == IPG_FP32, "precision must be IPG_FP64 or IPG_FP32");
   double rc = max_rcut(terms, nterms);
   DevList L;
@@ -315,26 +320,16 @@ static void pair_dev(ipg_ctx* ctx, const DevSystem& s, const ipg_pair_term* term
     L = *prebuilt;
     IPG_REQUIRE(L.n == s.n, "list natoms does not match the system");
   } else {
-    // one scan into neighbor rows of the width estimated from the density,
-    // used at once; the overflow bit is read after the pair kernel (one
-    // synchronization, and none in the middle of the call)
     int32_t* flag = scratch_t<int32_t>(ctx, "pair_list_flag", 1);
     zero(ctx, flag, sizeof(int32_t));
-    L = build_list(ctx, s, rc, KIND_FULL, false, false, "pair_list", 0, flag, -1, true);
+    L = build_list(ctx, s, rc, KIND_FULL, false, false, "pair_list", 0, flag, counted ? 0 : -1, !counted);
     if (L.deferred) {
       PairTable tab = prepare_terms(terms, nterms, list_rcut);
       double* ev = scratch_t<double>(ctx, "pair_ev", 8);
       double* f = d_forces ? d_forces : scratch_t<double>(ctx, "pair_fscratch", 3 * (s.n > 0 ? s.n : 1));
       run_pair(ctx, s, L, tab, precision, f, ev);
-      int32_t over = 0;
-      d2h(ctx, &over, flag, sizeof(int32_t));
-      sync(ctx);
-      if ((over & 8) == 0) {
-        launch(ctx, "split_ev", [&] { k_split_ev<<<1, 64, 0, ctx->stream>>>(ev, d_energy, d_virial); });
-        return;
-      }
-      // a row outgrew the width: a counted list, and the pair kernel again
-      L = build_list(ctx, s, rc, KIND_FULL, false, false, "pair_list", 0, flag, 0);
+      launch(ctx, "split_ev", [&] { k_split_ev<<<1, 64, 0, ctx->stream>>>(ev, d_energy, d_virial); });
+      return flag;
     }
   }
   // terms whose rcut is below the list's test their own membership
@@ -347,6 +342,18 @@ static void pair_dev(ipg_ctx* ctx, const DevSystem& s, const ipg_pair_term* term
     run_pair(ctx, s, L, tab, precision, f, ev);
   }
   launch(ctx, "split_ev", [&] { k_split_ev<<<1, 64, 0, ctx->stream>>>(ev, d_energy, d_virial); });
+  return nullptr;
+}
+
+// Device-buffer form: the flag is read here (one synchronization).
+static void pair_dev_checked(ipg_ctx* ctx, const DevSystem& s, const ipg_pair_term* terms, int nterms, int precision,
+                             double* d_energy, double* d_forces, double* d_virial) {
+  int32_t* flag = pair_dev(ctx, s, terms, nterms, precision, d_energy, d_forces, d_virial, nullptr, 0.0);
+  if (!flag) return;
+  int32_t over = 0;
+  d2h(ctx, &over, flag, sizeof(int32_t));
+  sync(ctx);
+  if (over & 8) pair_dev(ctx, s, terms, nterms, precision, d_energy, d_forces, d_virial, nullptr, 0.0, true);
 }
 
 }  // namespace ipg
@@ -365,11 +372,20 @@ int ipg_pair_energy_forces(ipg_ctx* ctx, const ipg_system* sys, const ipg_pair_t
     double* dev_e = dout;
     double* dev_v = dout + 1;
     double* dev_f = dout + 8;
-    pair_dev(ctx, s, terms, nterms, precision, dev_e, forces ? dev_f : nullptr, dev_v, nullptr, 0.0);
+    // results and the row-overflow flag come back in one synchronization;
+    // an overflow repeats the call with a counted list
     double ev[8];
-    d2h(ctx, ev, dout, sizeof(double) * 8);
-    if (forces) d2h(ctx, forces, dev_f, sizeof(double) * 3 * s.n);
-    sync(ctx);
+    int32_t over = 0;
+    for (int pass = 0; pass < 2; pass++) {
+      int32_t* flag = pair_dev(ctx, s, terms, nterms, precision, dev_e, forces ? dev_f : nullptr, dev_v, nullptr, 0.0,
+                               pass == 1);
+      d2h(ctx, ev, dout, sizeof(double) * 8);
+      if (forces) d2h(ctx, forces, dev_f, sizeof(double) * 3 * s.n);
+      over = 0;
+      if (flag) d2h(ctx, &over, flag, sizeof(int32_t));
+      sync(ctx);
+      if ((over & 8) == 0) break;
+    }
     if (energy) *energy = ev[0];
     if (virial)
       for (int k = 0; k < 6; k++) virial[k] = ev[1 + k];
@@ -382,7 +398,7 @@ int ipg_pair_energy_forces_dev(ipg_ctx* ctx, const ipg_system* sys, const ipg_pa
     IPG_REQUIRE(ctx && sys, "NULL argument");
     IPG_HIP(hipSetDevice(ctx->device));
     DevSystem s = make_dev_system(sys);
-    pair_dev(ctx, s, terms, nterms, precision, energy, forces, virial, nullptr, 0.0);
+    pair_dev_checked(ctx, s, terms, nterms, precision, energy, forces, virial);
   });
 }
 
