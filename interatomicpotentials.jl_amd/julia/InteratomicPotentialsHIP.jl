@@ -242,6 +242,23 @@ function InteratomicPotentials.compute_force_descriptors(s::AbstractSystem, h::H
     end
 end
 
+# snap.jl:226-285: (B, dB, W) with W the 6 x M virial rows summed over atoms.
+function InteratomicPotentials.compute_all_descriptors(s::AbstractSystem, h::HIP{<:SNAP}; mode=IPG_SNAP_LIST_REFERENCE)
+    snap = h.p
+    H = handle(snap)
+    M = H.ncoeff * length(snap.species)
+    appear = snap_appear(s)
+    with_system(s, ustrip, snap_species(s, snap)) do sys, X
+        B = zeros(Float64, H.ncoeff, length(s))
+        dB = zeros(Float64, 3, M, length(s))
+        W = zeros(Float64, 6, M)            # column c holds the library's row c of M x 6
+        check(ccall((:ipg_snap_all_descriptors, libipgpu), Cint,
+                    (Ptr{Cvoid}, Ref{IPGSystem}, Cint, Ptr{Int32}, Ptr{Float64}, Ptr{Float64}, Ptr{Float64}),
+                    H.ptr, sys, mode, appear, B, dB, W))
+        [B[:, i] for i in 1:length(s)], [dB[:, :, i] for i in 1:length(s)], W
+    end
+end
+
 function snap_energy_forces(s, lbp, path; mode=IPG_SNAP_LIST_REFERENCE)
     snap = lbp.basis
     H = handle(snap)
