@@ -77,6 +77,81 @@ __global__ void k_cell_rank(int64_t n, const int32_t* __restrict__ ctmp, const i
   sposf[dst] = make_float4((float)w[0], (float)w[1], (float)w[2], 0.f);
 }
 
+// The same binning in two launches, for small systems whose counts and slot
+// list fit in LDS: a thread per atom computes its cell and its double and FP32
+// coordinates (and clears the caller's flag word); then one block counts
+// the cells with LDS atomics, scans the counts, claims slots and ranks
+// every entry by atom index within its cell.  The outputs (cellof, cstart,
+// catom, spos, sposf) equal those of the four kernels above; a captured
+// step then waits on the host for two short launches instead of five.
+constexpr int CELLS_BLOCK = 1024;
+__global__ void k_cell_ids(int64_t n, const double* __restrict__ pos, const int32_t* __restrict__ species, Grid g,
+                           int32_t* __restrict__ cellof, double4* __restrict__ pw, float4* __restrict__ pf,
+                           int32_t* __restrict__ clear) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (clear && i == 0) *clear = 0;
+  if (i >= n) return;
+  const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+  const int c0 = cell_coord(g, 0, x), c1 = cell_coord(g, 1, y), c2 = cell_coord(g, 2, z);
+  cellof[i] = (int32_t)(((int64_t)c2 * g.nc[1] + c1) * g.nc[0] + c0);
+  pw[i] = make_double4(x, y, z, species ? (double)species[i] : 0.0);  // w: species index
+  double w[3] = {x, y, z};
+  for (int d = 0; d < 3; d++) w[d] = g.box.per[d] ? w[d] - floor(w[d] / g.box.L[d]) * g.box.L[d] : w[d] - g.lo[d];
+  pf[i] = make_float4((float)w[0], (float)w[1], (float)w[2], 0.f);
+}
+
+__global__ __launch_bounds__(CELLS_BLOCK) void k_cells_order(int32_t n, int32_t nc, const int32_t* __restrict__ cellof,
+                                                             const double4* __restrict__ pw,
+                                                             const float4* __restrict__ pf,
+                                                             int64_t* __restrict__ cstart, int32_t* __restrict__ catom,
+                                                             double4* __restrict__ spos, float4* __restrict__ sposf) {
+  extern __shared__ int32_t lds[];
+  __shared__ int32_t wsum[CELLS_BLOCK / 64];
+  int32_t* cur = lds;            // counts, then the end of every cell's slots
+  int32_t* slot = lds + nc + 1;  // atoms in slot order
+  const int tid = threadIdx.x;
+  for (int c = tid; c < nc; c += CELLS_BLOCK) cur[c] = 0;
+  __syncthreads();
+  for (int32_t i = tid; i < n; i += CELLS_BLOCK) atomicAdd(&cur[cellof[i]], 1);
+  __syncthreads();
+  // exclusive scan of the counts, each thread a contiguous run of cells
+  const int per = (nc + CELLS_BLOCK - 1) / CELLS_BLOCK;
+  const int c0 = min(tid * per, nc), c1 = min(c0 + per, nc);
+  int32_t own = 0;
+  for (int c = c0; c < c1; c++) own += cur[c];
+  const int lane = tid & 63, wv = tid >> 6;
+  int32_t x = own;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[wv] = x;
+  __syncthreads();
+  int32_t run = x - own;
+  for (int w = 0; w < wv; w++) run += wsum[w];
+  for (int c = c0; c < c1; c++) {
+    const int32_t k = cur[c];
+    cstart[c] = run;
+    cur[c] = run;  // fill cursor: the cell's first slot
+    run += k;
+  }
+  if (tid == 0) cstart[nc] = n;
+  __syncthreads();
+  for (int32_t i = tid; i < n; i += CELLS_BLOCK) slot[atomicAdd(&cur[cellof[i]], 1)] = i;
+  __syncthreads();
+  // cur[c] is now the end of cell c and the start of cell c + 1
+  for (int32_t k = tid; k < n; k += CELLS_BLOCK) {
+    const int32_t i = slot[k];
+    const int32_t c = cellof[i];
+    const int32_t a = c > 0 ? cur[c - 1] : 0, b = cur[c];
+    int32_t r = 0;
+    for (int32_t m = a; m < b; m++) r += slot[m] < i;
+    catom[a + r] = i;
+    spos[a + r] = pw[i];
+    sposf[a + r] = pf[i];
+  }
+}
+
 // Candidate atoms of a cell: the 3x3x3 block of cells around it (all cells
 // of a periodic dimension with fewer than 3 cells, clipped at open
 // boundaries), each visited once.  Cells are numbered x fastest and atoms
@@ -501,16 +576,32 @@ static DevList build_list_impl(ipg_ctx* ctx, const DevSystem& s, double rcut, in
   double4* spos = scratch_t<double4>(ctx, "nl_spos", s.n);
   float4* sposf = scratch_t<float4>(ctx, "nl_sposf", s.n);
   int32_t* cnt = scratch_t<int32_t>(ctx, "nl_cnt", s.n);
-  // the cell counters, and the caller's flag word when this build starts
-  // its step (one fill launch for both)
-  zero2(ctx, ccount, sizeof(int32_t) * (2 * g.ncell + 2), capflag, clear_flag ? sizeof(int32_t) : 0);
   const unsigned gb = grid_for(s.n, 256);
-  launch(ctx, "cell_assign", [&] { k_cell_assign<<<gb, 256, 0, ctx->stream>>>(s.pos, s.n, g, cellof, ccount); });
-  exclusive_scan(ctx, ccount, g.ncell, cstart, false);
-  launch(ctx, "cell_place", [&] { k_cell_place<<<gb, 256, 0, ctx->stream>>>(s.n, cellof, cstart, cfill, ctmp); });
-  launch(ctx, "cell_rank", [&] {
-    k_cell_rank<<<gb, 256, 0, ctx->stream>>>(s.n, ctmp, cellof, cstart, s.pos, s.species, g, catom, spos, sposf);
-  });
+  const size_t cells_lds = sizeof(int32_t) * (size_t)(s.n + g.ncell + 1);
+  // the ordering block is latency-bound on one CU: 8192 atoms took 27 us
+  // there against about 20 us for the four kernels, 1000 atoms a few us
+  if (s.n <= 4096 && cells_lds <= 48 * 1024) {
+    double4* pw = scratch_t<double4>(ctx, "nl_pw", s.n);
+    float4* pfw = scratch_t<float4>(ctx, "nl_pf", s.n);
+    launch(ctx, "cell_assign", [&] {
+      k_cell_ids<<<gb, 256, 0, ctx->stream>>>(s.n, s.pos, s.species, g, cellof, pw, pfw,
+                                              clear_flag ? capflag : nullptr);
+    });
+    launch(ctx, "cell_rank", [&] {
+      k_cells_order<<<1, CELLS_BLOCK, cells_lds, ctx->stream>>>((int32_t)s.n, (int32_t)g.ncell, cellof, pw, pfw,
+                                                                cstart, catom, spos, sposf);
+    });
+  } else {
+    // the cell counters, and the caller's flag word when this build starts
+    // its step (one fill launch for both)
+    zero2(ctx, ccount, sizeof(int32_t) * (2 * g.ncell + 2), capflag, clear_flag ? sizeof(int32_t) : 0);
+    launch(ctx, "cell_assign", [&] { k_cell_assign<<<gb, 256, 0, ctx->stream>>>(s.pos, s.n, g, cellof, ccount); });
+    exclusive_scan(ctx, ccount, g.ncell, cstart, false);
+    launch(ctx, "cell_place", [&] { k_cell_place<<<gb, 256, 0, ctx->stream>>>(s.n, cellof, cstart, cfill, ctmp); });
+    launch(ctx, "cell_rank", [&] {
+      k_cell_rank<<<gb, 256, 0, ctx->stream>>>(s.n, ctmp, cellof, cstart, s.pos, s.species, g, catom, spos, sposf);
+    });
+  }
   // lanes per atom from the expected candidates per contiguous row range
   const double cand = (double)s.n / (double)g.ncell * (g.full[0] ? g.nc[0] : std::min(3, g.nc[0]));
   const int LP = cand >= 80 ? 64 : (cand >= 18 ? 32 : (cand >= 9 ? 16 : 8));
