@@ -94,10 +94,48 @@ __global__ void k_cell_ids(int64_t n, const double* __restrict__ pos, const int3
   const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
   const int c0 = cell_coord(g, 0, x), c1 = cell_coord(g, 1, y), c2 = cell_coord(g, 2, z);
   cellof[i] = (int32_t)(((int64_t)c2 * g.nc[1] + c1) * g.nc[0] + c0);
+  if (!pw) return;  // cells only
   pw[i] = make_double4(x, y, z, species ? (double)species[i] : 0.0);  // w: species index
   double w[3] = {x, y, z};
   for (int d = 0; d < 3; d++) w[d] = g.box.per[d] ? w[d] - floor(w[d] / g.box.L[d]) * g.box.L[d] : w[d] - g.lo[d];
   pf[i] = make_float4((float)w[0], (float)w[1], (float)w[2], 0.f);
+}
+
+// Mid-size systems: the block only counts, scans and claims slots (ctmp, in
+// global memory); the parallel k_cell_rank then orders and gathers.
+__global__ __launch_bounds__(CELLS_BLOCK) void k_cells_slots(int32_t n, int32_t nc, const int32_t* __restrict__ cellof,
+                                                             int64_t* __restrict__ cstart, int32_t* __restrict__ ctmp) {
+  extern __shared__ int32_t lds[];
+  __shared__ int32_t wsum[CELLS_BLOCK / 64];
+  int32_t* cur = lds;
+  const int tid = threadIdx.x;
+  for (int c = tid; c < nc; c += CELLS_BLOCK) cur[c] = 0;
+  __syncthreads();
+  for (int32_t i = tid; i < n; i += CELLS_BLOCK) atomicAdd(&cur[cellof[i]], 1);
+  __syncthreads();
+  const int per = (nc + CELLS_BLOCK - 1) / CELLS_BLOCK;
+  const int c0 = min(tid * per, nc), c1 = min(c0 + per, nc);
+  int32_t own = 0;
+  for (int c = c0; c < c1; c++) own += cur[c];
+  const int lane = tid & 63, wv = tid >> 6;
+  int32_t x = own;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[wv] = x;
+  __syncthreads();
+  int32_t run = x - own;
+  for (int w = 0; w < wv; w++) run += wsum[w];
+  for (int c = c0; c < c1; c++) {
+    const int32_t k = cur[c];
+    cstart[c] = run;
+    cur[c] = run;
+    run += k;
+  }
+  if (tid == 0) cstart[nc] = n;
+  __syncthreads();
+  for (int32_t i = tid; i < n; i += CELLS_BLOCK) ctmp[atomicAdd(&cur[cellof[i]], 1)] = i;
 }
 
 __global__ __launch_bounds__(CELLS_BLOCK) void k_cells_order(int32_t n, int32_t nc, const int32_t* __restrict__ cellof,
@@ -590,6 +628,20 @@ static DevList build_list_impl(ipg_ctx* ctx, const DevSystem& s, double rcut, in
     launch(ctx, "cell_rank", [&] {
       k_cells_order<<<1, CELLS_BLOCK, cells_lds, ctx->stream>>>((int32_t)s.n, (int32_t)g.ncell, cellof, pw, pfw,
                                                                 cstart, catom, spos, sposf);
+    });
+  } else if (s.n <= 16384 && sizeof(int32_t) * (size_t)(g.ncell + 1) <= 48 * 1024) {
+    // three launches: cells (and the flag clear), slots in one block, then
+    // the parallel rank and gather
+    launch(ctx, "cell_assign", [&] {
+      k_cell_ids<<<gb, 256, 0, ctx->stream>>>(s.n, s.pos, s.species, g, cellof, nullptr, nullptr,
+                                              clear_flag ? capflag : nullptr);
+    });
+    launch(ctx, "cell_place", [&] {
+      k_cells_slots<<<1, CELLS_BLOCK, sizeof(int32_t) * (size_t)(g.ncell + 1), ctx->stream>>>(
+          (int32_t)s.n, (int32_t)g.ncell, cellof, cstart, ctmp);
+    });
+    launch(ctx, "cell_rank", [&] {
+      k_cell_rank<<<gb, 256, 0, ctx->stream>>>(s.n, ctmp, cellof, cstart, s.pos, s.species, g, catom, spos, sposf);
     });
   } else {
     // the cell counters, and the caller's flag word when this build starts
