@@ -106,18 +106,20 @@ def test_large_periodic_lattice(ipg, orc):
     assert 37.0 < len(a[1]) / 1000 < 41.0
 
 
-@pytest.mark.parametrize("n", [3000, 9000, 20000])
-def test_binning_paths_by_size(ipg, orc, n):
+@pytest.mark.parametrize("n,per", [(3000, (True, True, True)), (9000, (True, True, True)),
+                                   (9000, (True, False, True)), (20000, (True, True, True))])
+def test_binning_paths_by_size(ipg, orc, n, per):
     """The cell binning runs in two launches up to 4096 atoms, three up to
     16384 and five above; every path gives the oracle's lists exactly (half
-    and full, periodic box at liquid-like density)."""
+    and full, liquid-like density, periodic or mixed boundaries)."""
     rng = np.random.default_rng(n)
     L = (n / 0.05) ** (1.0 / 3.0)
     pos = rng.random((n, 3)) * L
     box = [L, L, L]
     for full in (False, True):
-        a = gpu_list(ipg, pos, box, (True,) * 3, 5.0, full=full)
-        b = orc.neighborlist(pos, box, 5.0, kind=orc.LIST_FULL if full else orc.LIST_HALF, cells=True)
+        a = gpu_list(ipg, pos, box, per, 5.0, full=full)
+        Lo = [box[d] if per[d] else INF for d in range(3)]
+        b = orc.neighborlist(pos, Lo, 5.0, kind=orc.LIST_FULL if full else orc.LIST_HALF, cells=True)
         assert_same(a, b)
         assert len(a[1]) > (10 if full else 5) * n
 
