@@ -13,8 +13,16 @@
  *    synchronous).  The caller owns all host buffers; the library keeps no
  *    host pointer after return.
  *  - *_dev entry points take device pointers for every array (inputs and
- *    outputs), enqueue work on the context's stream and return without
- *    synchronizing; ipg_ctx_synchronize() waits for them.
+ *    outputs) and enqueue work on the context's stream.  Those whose
+ *    comment says so synchronize the stream once, at their end, to read a
+ *    status word (a list outgrew its buffer, a species index is out of
+ *    range); the others return without synchronizing, and
+ *    ipg_ctx_synchronize() waits for them.
+ *  - Species indices must lie in [0, 64) for pair potentials and in
+ *    [0, nspecies) for SNAP.  Host entry points check them before any
+ *    work; device entry points check them on the device and return
+ *    IPG_ERR_INVALID (the outputs of that call are then unspecified, and no
+ *    kernel reads outside its tables).
  *  - Atom indices are 0-based (the Julia wrapper adds 1).
  *  - Positions are AoS x,y,z per atom (Julia Vector{SVector{3,Float64}}
  *    reinterpreted).  Boxes are orthorhombic: box[d] is bounding_box(s)[d][d].
@@ -158,11 +166,15 @@ typedef struct {
 int ipg_pair_energy_forces(ipg_ctx* ctx, const ipg_system* sys, const ipg_pair_term* terms, int nterms,
                            int precision, double* energy, double* forces, double* virial);
 /* Device variant: every pointer (sys arrays, energy[1], forces[3N],
- * virial[6]) is a device pointer; terms is a host array. */
+ * virial[6]) is a device pointer; terms is a host array.  Synchronizes once
+ * at its end to read the status word (neighbor-row overflow, which repeats
+ * the evaluation with a counted list, and out-of-range species). */
 int ipg_pair_energy_forces_dev(ipg_ctx* ctx, const ipg_system* sys, const ipg_pair_term* terms, int nterms,
                                int precision, double* energy, double* forces, double* virial);
 /* Pair kernel alone over a prebuilt device full list (IPG_LIST_FULL at a
- * cutoff >= every term's rcut); pointers as in the _dev variant. */
+ * cutoff >= every term's rcut); pointers as in the _dev variant.
+ * Synchronizes once at its end to read the status word (out-of-range
+ * species). */
 int ipg_pair_eval_dev(ipg_ctx* ctx, const ipg_nlist* full_list, const ipg_system* sys, const ipg_pair_term* terms,
                       int nterms, int precision, double* energy, double* forces, double* virial);
 
@@ -222,6 +234,24 @@ int ipg_snap_all_descriptors(ipg_snap* snap, const ipg_system* sys, int mode, co
 int ipg_snap_energy_forces(ipg_snap* snap, const ipg_system* sys, int mode, const int32_t* appear,
                            const double* beta, int64_t nbeta, double beta0, int path, double* energy,
                            double* forces);
+/* Replaces virial_stress(s, ::BasisPotential) and virial(s, ...)
+ * (src/BasisPotentials/basis_potentials.jl:43-57) for the linear potential,
+ * W * beta (linear_potential.jl:37-49) with W of compute_all_descriptors
+ * (snap.jl:267-281: raw positions, rows [xx, yy, zz, zy, zx, yx]); the
+ * reference's own call passes the potential where compute_virial_descriptors
+ * expects the basis (a MethodError), so this is its intended result.
+ * virial[6]; virial(s, lbp) is the sum of the six entries.  path as in
+ * ipg_snap_energy_forces: the plain route forms W from the per-pair dB and
+ * contracts it with beta on the device; the adjoint routes contract
+ * dE_i/dr_ij = beta_s(i) . dB_i/dr_ij with x_i - x_j directly. */
+int ipg_snap_virial_stress(ipg_snap* snap, const ipg_system* sys, int mode, const int32_t* appear, const double* beta,
+                           int64_t nbeta, int path, double* virial);
+/* Energy, forces and virial stress of the linear SNAP potential from one
+ * evaluation (ipg_snap_energy_forces and ipg_snap_virial_stress in one
+ * pass over the per-pair derivatives).  Any output may be NULL. */
+int ipg_snap_energy_forces_virial(ipg_snap* snap, const ipg_system* sys, int mode, const int32_t* appear,
+                                  const double* beta, int64_t nbeta, double beta0, int path, double* energy,
+                                  double* forces, double* virial);
 /* Device variant: sys arrays, beta (ncoeff*nspecies, expanded), energy[1]
  * and forces[3N] are device pointers.  For a fully periodic system the
  * step is captured into a HIP graph on the first call and replayed while
@@ -264,7 +294,7 @@ int ipg_snap_batch_descriptors(ipg_snap* snap, int64_t nconf, const ipg_system* 
 /* Device variant: the layout (natoms[nconf], box[nconf][3],
  * periodic[nconf][3]) is on the host; pos (3*natoms_total), species
  * (natoms_total, may be NULL) and the outputs are device pointers in the
- * packed order above.  Species indices are not checked on the device.  The
+ * packed order above.  Species indices are checked on the device.  The
  * call synchronizes the context stream once at its end (it reads the
  * neighbor-list flag word and recomputes exactly on a capacity overflow). */
 int ipg_snap_batch_descriptors_dev(ipg_snap* snap, int64_t nconf, const int64_t* natoms, const double* box,

@@ -5,6 +5,30 @@
 #include <algorithm>
 
 #include <cfloat>
+#include <cstddef>
+
+// The Julia binding (julia/InteratomicPotentialsHIP.jl: IPGSystem,
+// IPGPairTerm, IPGSnapParams) and the Python mirror (ipgpu/_lib.py) lay
+// these structs out by hand with natural alignment; a change here must be
+// made there too.
+static_assert(sizeof(ipg_system) == 64, "ipg_system size");
+static_assert(offsetof(ipg_system, natoms) == 0 && offsetof(ipg_system, positions) == 8 &&
+                  offsetof(ipg_system, species) == 16 && offsetof(ipg_system, box) == 24 &&
+                  offsetof(ipg_system, periodic) == 48,
+              "ipg_system layout");
+static_assert(sizeof(ipg_pair_term) == 80, "ipg_pair_term size");
+static_assert(offsetof(ipg_pair_term, kind) == 0 && offsetof(ipg_pair_term, reserved) == 4 &&
+                  offsetof(ipg_pair_term, coeff) == 8 && offsetof(ipg_pair_term, rcut) == 16 &&
+                  offsetof(ipg_pair_term, p) == 24 && offsetof(ipg_pair_term, species_mask) == 72,
+              "ipg_pair_term layout");
+static_assert(sizeof(ipg_snap_params) == 72, "ipg_snap_params size");
+static_assert(offsetof(ipg_snap_params, twojmax) == 0 && offsetof(ipg_snap_params, nspecies) == 4 &&
+                  offsetof(ipg_snap_params, rcutfac) == 8 && offsetof(ipg_snap_params, rmin0) == 16 &&
+                  offsetof(ipg_snap_params, rfac0) == 24 && offsetof(ipg_snap_params, radii) == 32 &&
+                  offsetof(ipg_snap_params, weight) == 40 && offsetof(ipg_snap_params, chem_flag) == 48 &&
+                  offsetof(ipg_snap_params, bzero_flag) == 52 && offsetof(ipg_snap_params, bnorm_flag) == 56 &&
+                  offsetof(ipg_snap_params, switch_flag) == 60 && offsetof(ipg_snap_params, wselfall_flag) == 64,
+              "ipg_snap_params layout");
 
 namespace ipg {
 

@@ -306,7 +306,10 @@ __device__ __forceinline__ bool snap_cut_ok(const IpgBox& b, const double4& pi, 
   const double ry = -1.0 * ipg_disp1(pi.y, pj.y, b.L[1], b.per[1]);
   const double rz = -1.0 * ipg_disp1(pi.z, pj.z, b.L[2], b.per[2]);
   const double rsq = rx * rx + ry * ry + rz * rz;
-  return rsq <= sc.cutsq[(int)pi.w * sc.ns + (int)pj.w];
+  // species indices come from the w components; one outside [0, ns) reads
+  // species 0 (the SNAP kernels report it)
+  const int si = (int)pi.w, sj = (int)pj.w;
+  return rsq <= sc.cutsq[((unsigned)si < (unsigned)sc.ns ? si : 0) * sc.ns + ((unsigned)sj < (unsigned)sc.ns ? sj : 0)];
 }
 
 struct Prefilter {
@@ -796,6 +799,7 @@ __global__ void k_snap_stage_b(int64_t n, const int64_t* __restrict__ offA, cons
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int ne = cut.reference ? appear[i] : (species ? species[i] : 0);
+  ne = (unsigned)ne < (unsigned)cut.ns ? ne : 0;  // an index outside [0, ns) reads species 0
   double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
   int64_t a = offA[i], b = offA[i + 1];
   int count = 0;
@@ -804,6 +808,7 @@ __global__ void k_snap_stage_b(int64_t n, const int64_t* __restrict__ offA, cons
     int32_t j = nbrA[q];
     int64_t m = q - a + 1;  // 1-based enumerate index (utl.jl:75)
     int me = cut.reference ? appear[m - 1] : (species ? species[j] : 0);
+    me = (unsigned)me < (unsigned)cut.ns ? me : 0;
     double rx = -1.0 * ipg_disp1(xi, pos[3 * j], box.L[0], box.per[0]);
     double ry = -1.0 * ipg_disp1(yi, pos[3 * j + 1], box.L[1], box.per[1]);
     double rz = -1.0 * ipg_disp1(zi, pos[3 * j + 2], box.L[2], box.per[2]);

@@ -122,7 +122,8 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 6 : 1) void k_pair(int64_t n,
                                               const int32_t* __restrict__ nbr, const int32_t* __restrict__ cnt,
                                               int K, const double* __restrict__ pos,
                                               const int32_t* __restrict__ species, IpgBox box, PairTable tab,
-                                              double* __restrict__ forces, double* __restrict__ part) {
+                                              double* __restrict__ forces, double* __restrict__ part,
+                                              int32_t* __restrict__ flag) {
   const int tid = threadIdx.x;
   const int64_t a = ((int64_t)blockIdx.x * 256 + tid) / LANES;
   const int lane = tid % LANES;
@@ -131,6 +132,9 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 6 : 1) void k_pair(int64_t n,
   if (a < n) {
     const double xa = pos[3 * a], ya = pos[3 * a + 1], za = pos[3 * a + 2];
     const int sa = species ? species[a] : 0;
+    // a species index outside [0, 64) belongs to no term's species set
+    // (it contributes nothing) and raises bit 16 for the caller
+    if ((unsigned)sa >= 64u) atomicOr(flag, 16);
     const int64_t q0 = ROWS ? a * K : off[a], q1 = ROWS ? a * K + cnt[a] : off[a + 1];
     for (int64_t q = q0 + lane; q < q1; q += LANES) {
       const int32_t j = nbr[q];
@@ -150,7 +154,8 @@ __global__ __launch_bounds__(256, sizeof(T) == 4 ? 6 : 1) void k_pair(int64_t n,
       T vsum = 0, fsum = 0;
       for (int k = 0; k < tab.nterms; k++) {
         const PairTermDev& t = tab.t[k];
-        if (!((t.mask >> sa) & 1ull) || !((t.mask >> sj) & 1ull)) continue;
+        if ((unsigned)sa >= 64u || (unsigned)sj >= 64u || !((t.mask >> sa) & 1ull) || !((t.mask >> sj) & 1ull))
+          continue;
         if (t.check && !(spe <= t.smax)) continue;
         T fs;
         T vv = pair_eval<T>(t, R, ir, fs);
@@ -273,7 +278,7 @@ static int pick_lanes(int64_t n, int64_t npairs) {
 
 // Launch the pair kernel over a full list; e/v to d_ev[7] (device).
 static void run_pair(ipg_ctx* ctx, const DevSystem& s, const DevList& L, const PairTable& tab, int precision,
-                     double* d_forces, double* d_ev) {
+                     double* d_forces, double* d_ev, int32_t* flag) {
   int lanes = pick_lanes(s.n, L.npairs);
   int64_t nthreads = s.n * lanes;
   int64_t nblk = (nthreads + 255) / 256;
@@ -286,10 +291,10 @@ static void run_pair(ipg_ctx* ctx, const DevSystem& s, const DevList& L, const P
     launch(ctx, nm, [&] {
       if (L.ellK > 0)
         k_pair<T, LN, true><<<(unsigned)nblk, 256, 0, ctx->stream>>>(s.n, nullptr, L.ell, L.cnt, L.ellK, s.pos,
-                                                                     s.species, s.box, tab, d_forces, part);
+                                                                     s.species, s.box, tab, d_forces, part, flag);
       else
         k_pair<T, LN, false><<<(unsigned)nblk, 256, 0, ctx->stream>>>(s.n, L.off, L.nbr, nullptr, 0, s.pos,
-                                                                      s.species, s.box, tab, d_forces, part);
+                                                                      s.species, s.box, tab, d_forces, part, flag);
     });
   };
   if (precision == IPG_FP64) {
@@ -306,9 +311,10 @@ static void run_pair(ipg_ctx* ctx, const DevSystem& s, const DevList& L, const P
 
 // Without a prebuilt list the call scans into neighbor rows of the width
 // estimated from the density and uses them at once; bit 8 of the returned
-// flag word (null when there is nothing to check) says a row outgrew that
-// width, and the call must then be repeated with counted = true.  The flag
-// is read by the caller, together with its results (one synchronization).
+// flag word says a row outgrew that width, and the call must then be
+// repeated with counted = true.  Bit 16 says a species index was outside
+// [0, 64) (such an atom took part in no term).  The flag is read by the
+// caller, together with its results (one synchronization).
 static int32_t* pair_dev(ipg_ctx* ctx, const DevSystem& s, const ipg_pair_term* terms, int nterms, int precision,
                          double* d_energy, double* d_forces, double* d_virial, const DevList* prebuilt,
                          double prebuilt_rcut, bool counted = false) {
@@ -316,18 +322,18 @@ static int32_t* pair_dev(ipg_ctx* ctx, const DevSystem& s, const ipg_pair_term* 
   double rc = max_rcut(terms, nterms);
   DevList L;
   double list_rcut = prebuilt ? prebuilt_rcut : rc;
+  int32_t* flag = scratch_t<int32_t>(ctx, "pair_list_flag", 1);
+  zero(ctx, flag, sizeof(int32_t));
   if (prebuilt) {
     L = *prebuilt;
     IPG_REQUIRE(L.n == s.n, "list natoms does not match the system");
   } else {
-    int32_t* flag = scratch_t<int32_t>(ctx, "pair_list_flag", 1);
-    zero(ctx, flag, sizeof(int32_t));
     L = build_list(ctx, s, rc, KIND_FULL, false, false, "pair_list", 0, flag, counted ? 0 : -1, !counted);
     if (L.deferred) {
       PairTable tab = prepare_terms(terms, nterms, list_rcut);
       double* ev = scratch_t<double>(ctx, "pair_ev", 8);
       double* f = d_forces ? d_forces : scratch_t<double>(ctx, "pair_fscratch", 3 * (s.n > 0 ? s.n : 1));
-      run_pair(ctx, s, L, tab, precision, f, ev);
+      run_pair(ctx, s, L, tab, precision, f, ev, flag);
       launch(ctx, "split_ev", [&] { k_split_ev<<<1, 64, 0, ctx->stream>>>(ev, d_energy, d_virial); });
       return flag;
     }
@@ -339,20 +345,24 @@ static int32_t* pair_dev(ipg_ctx* ctx, const DevSystem& s, const ipg_pair_term* 
   if (s.n == 0) {
     zero(ctx, ev, sizeof(double) * 8);
   } else {
-    run_pair(ctx, s, L, tab, precision, f, ev);
+    run_pair(ctx, s, L, tab, precision, f, ev, flag);
   }
   launch(ctx, "split_ev", [&] { k_split_ev<<<1, 64, 0, ctx->stream>>>(ev, d_energy, d_virial); });
-  return nullptr;
+  return flag;
+}
+
+static void check_pair_flag(int32_t over) {
+  if (over & 16) fail(IPG_ERR_INVALID, "species index out of range [0, 64)");
 }
 
 // Device-buffer form: the flag is read here (one synchronization).
 static void pair_dev_checked(ipg_ctx* ctx, const DevSystem& s, const ipg_pair_term* terms, int nterms, int precision,
                              double* d_energy, double* d_forces, double* d_virial) {
   int32_t* flag = pair_dev(ctx, s, terms, nterms, precision, d_energy, d_forces, d_virial, nullptr, 0.0);
-  if (!flag) return;
   int32_t over = 0;
   d2h(ctx, &over, flag, sizeof(int32_t));
   sync(ctx);
+  check_pair_flag(over);
   if (over & 8) pair_dev(ctx, s, terms, nterms, precision, d_energy, d_forces, d_virial, nullptr, 0.0, true);
 }
 
@@ -382,8 +392,9 @@ int ipg_pair_energy_forces(ipg_ctx* ctx, const ipg_system* sys, const ipg_pair_t
       d2h(ctx, ev, dout, sizeof(double) * 8);
       if (forces) d2h(ctx, forces, dev_f, sizeof(double) * 3 * s.n);
       over = 0;
-      if (flag) d2h(ctx, &over, flag, sizeof(int32_t));
+      d2h(ctx, &over, flag, sizeof(int32_t));
       sync(ctx);
+      check_pair_flag(over);
       if ((over & 8) == 0) break;
     }
     if (energy) *energy = ev[0];
@@ -410,7 +421,12 @@ int ipg_pair_eval_dev(ipg_ctx* ctx, const ipg_nlist* full_list, const ipg_system
     IPG_REQUIRE(max_rcut(terms, nterms) <= full_list->cut, "list cutoff is below a term's rcut");
     IPG_HIP(hipSetDevice(ctx->device));
     DevSystem s = make_dev_system(sys);
-    pair_dev(ctx, s, terms, nterms, precision, energy, forces, virial, &full_list->list, full_list->rcut);
+    int32_t* flag =
+        pair_dev(ctx, s, terms, nterms, precision, energy, forces, virial, &full_list->list, full_list->rcut);
+    int32_t over = 0;
+    d2h(ctx, &over, flag, sizeof(int32_t));
+    sync(ctx);
+    check_pair_flag(over);
   });
 }
 

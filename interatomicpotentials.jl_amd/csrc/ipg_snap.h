@@ -21,7 +21,7 @@ struct YBlk {
   int jjb[3];
   double fac[3];
 };
-// The three half-sums of dB_jjb/dr (compute_b.jl:207-402): level, Z block
+// The three half-sums of dB_jjb/dr (compute_b.jl:145-337): level, Z block
 // start, factor.
 struct BTerm {
   int l[3];
@@ -94,7 +94,18 @@ struct SnapRun {
   double avg_nbr = -1.0;          // mean neighbors per atom for launch heuristics (<0: npairs/n)
   int apw = 0;                    // U kernel atoms per wave (1 or 2; 0: from avg_nbr)
   int dbdr_threads = 0;           // dB kernel block size (64 or 128; 0: 128)
+  int32_t* flag = nullptr;        // call's flag word: the U kernel raises bit 16 for a species index
+                                  // outside [0, nspecies)
 };
+
+// Species index of atom i clamped into [0, ns).  Device entry points take
+// species arrays unchecked, so no kernel lets a bad index address a table;
+// the U kernel reports it (bit 16 of the call's flag word) and the call
+// fails.
+__device__ __forceinline__ int snap_species(const int32_t* sp, int64_t i, int ns) {
+  const int s = sp ? sp[i] : 0;
+  return (unsigned)s < (unsigned)ns ? s : 0;
+}
 
 // ---------------------------------------------------------------------------
 // Dual numbers: value + tangent, for exact forward-mode directional

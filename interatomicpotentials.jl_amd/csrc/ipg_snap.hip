@@ -141,7 +141,7 @@ void snap_build_host(SnapHost& h, const ipg_snap_params* p) {
     for (int k = 0; k < y.n; k++) IPG_REQUIRE(y.jjb[k] >= 0, "internal: bad Y block");
     h.yblk.push_back(y);
   }
-  // dB/dr half-sum terms per triple (compute_b.jl:207-399)
+  // dB/dr half-sum terms per triple (compute_b.jl:145-337)
   h.bterm.clear();
   for (int jjb = 0; jjb < h.idxb_max; jjb++) {
     const int j1 = h.idxb[jjb][0], j2 = h.idxb[jjb][1], j = h.idxb[jjb][2];
@@ -343,7 +343,7 @@ template <typename T>
 __global__ void k_db_gather(SnapDev P, SnapRun R, const T* __restrict__ dblist, double* __restrict__ dB) {
   const int64_t a = blockIdx.x;
   const int M = P.ncoeff * P.nspecies;
-  const int sa = R.species ? R.species[a] : 0;
+  const int sa = snap_species(R.species, a, P.nspecies);
   const int64_t q0 = R.off[a], q1 = R.off[a + 1], r0 = R.roff[a], r1 = R.roff[a + 1];
   for (int t = threadIdx.x; t < 3 * P.ncoeff; t += blockDim.x) {
     double acc[8];
@@ -356,7 +356,7 @@ __global__ void k_db_gather(SnapDev P, SnapRun R, const T* __restrict__ dblist, 
       if (s == sa) acc[s] = own;
     for (int64_t r = r0; r < r1; r++) {
       const int64_t q = R.rpair[r];
-      const int sc = R.species ? R.species[R.center[q]] : 0;
+      const int sc = snap_species(R.species, R.center[q], P.nspecies);
       const double v = (double)dblist[q * 3 * P.ncoeff + t];
 #pragma unroll
       for (int s = 0; s < 8; s++)
@@ -380,7 +380,7 @@ __global__ void k_db_gather_sym(SnapDev P, SnapRun R, const T* __restrict__ dbli
                                 const double* __restrict__ pos, double* __restrict__ Wa) {
   const int64_t a = blockIdx.x;
   const int M = P.ncoeff * P.nspecies, nc3 = 3 * P.ncoeff;
-  const int sa = R.species ? R.species[a] : 0;
+  const int sa = snap_species(R.species, a, P.nspecies);
   const int64_t q0 = R.off[a], q1 = R.off[a + 1];
   const double xa = Wa ? pos[3 * a] : 0.0, ya = Wa ? pos[3 * a + 1] : 0.0, za = Wa ? pos[3 * a + 2] : 0.0;
   for (int t = threadIdx.x; t < nc3; t += blockDim.x) {
@@ -417,7 +417,7 @@ __global__ void k_db_gather_sym(SnapDev P, SnapRun R, const T* __restrict__ dbli
     for (int s = 0; s < 8; s++) acc[s] = s == sa ? own : 0.0;
 #pragma unroll 8
     for (int64_t q = q0; q < q1; q++) {
-      const int sc = R.species ? R.species[R.nbr[q]] : 0;
+      const int sc = snap_species(R.species, R.nbr[q], P.nspecies);
       const double v = (double)dblist[(int64_t)R.rev[q] * nc3 + t];
 #pragma unroll
       for (int s = 0; s < 8; s++)
@@ -449,7 +449,7 @@ __global__ void k_w_atom(SnapDev P, SnapRun R, const double* __restrict__ pos, c
   if (t >= R.n * (int64_t)M) return;
   const int64_t a = t / M;
   const int m = (int)(t % M);
-  const int sa = R.species ? R.species[a] : 0;
+  const int sa = snap_species(R.species, a, P.nspecies);
   double w[6] = {0, 0, 0, 0, 0, 0};
   if (m / P.ncoeff == sa) {
     const int c = m % P.ncoeff;
@@ -467,6 +467,42 @@ __global__ void k_w_atom(SnapDev P, SnapRun R, const double* __restrict__ pos, c
     }
   }
   for (int v = 0; v < 6; v++) Wa[t * 6 + v] = w[v];
+}
+
+// Per-center-atom virial of the adjoint route: beta_s(i) . W over the
+// atom's own pairs is sum_q dE_i/dr_ij (x) (x_i - x_j) with raw positions
+// (snap.jl:267-281 contracted with beta, linear_potential.jl:37-42), rows
+// [xx, yy, zz, zy, zx, yx] with the first index the dE/dr component.
+__global__ void k_virial_atoms(int64_t n, const int64_t* __restrict__ off, const int32_t* __restrict__ nbr,
+                               const double* __restrict__ pos, const double* __restrict__ dedr,
+                               double* __restrict__ va) {
+  const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  const double xi = pos[3 * a], yi = pos[3 * a + 1], zi = pos[3 * a + 2];
+  double w[6] = {0, 0, 0, 0, 0, 0};
+  for (int64_t q = off[a]; q < off[a + 1]; q++) {
+    const int32_t j = nbr[q];
+    const double dx = xi - pos[3 * j], dy = yi - pos[3 * j + 1], dz = zi - pos[3 * j + 2];
+    const double gx = dedr[3 * q], gy = dedr[3 * q + 1], gz = dedr[3 * q + 2];
+    w[0] += gx * dx;
+    w[1] += gy * dy;
+    w[2] += gz * dz;
+    w[3] += gy * dz;
+    w[4] += gx * dz;
+    w[5] += gx * dy;
+  }
+  for (int v = 0; v < 6; v++) va[a * 6 + v] = w[v];
+}
+
+// virial[v] = sum_m W[m][v] beta[m] (linear_potential.jl:37-42, W * beta),
+// one lane per row, terms in m order.
+__global__ void k_w_beta(int M, const double* __restrict__ W, const double* __restrict__ beta,
+                         double* __restrict__ virial) {
+  const int v = threadIdx.x;
+  if (v >= 6) return;
+  double s = 0.0;
+  for (int m = 0; m < M; m++) s += W[m * 6 + v] * beta[m];
+  virial[v] = s;
 }
 
 __global__ void k_finish_energy(const double* __restrict__ sum, double beta0, double* __restrict__ out) {
@@ -529,6 +565,7 @@ static SnapCall snap_prepare(ipg_snap* sn, const DevSystem& s, int mode, const i
   c.R.roff = c.L.roff;
   c.R.rpair = c.L.rpair;
   c.R.rev = c.L.rev;
+  c.R.flag = c.flag;
   return c;
 }
 
@@ -615,8 +652,12 @@ static void gather_db(ipg_snap* sn, const SnapCall& c, const double* dbl, double
 }
 
 // Energy and forces on the device; d_energy/d_forces are device pointers.
+// d_virial (optional, device, 6): beta . W, the virial stress of the
+// linear potential, from the same per-pair derivatives; pos holds the raw
+// positions it needs.
 static void energy_forces_dev(ipg_snap* sn, const SnapCall& c, const double* d_beta, double beta0, int path,
-                              double* d_energy, double* d_forces) {
+                              double* d_energy, double* d_forces, double* d_virial = nullptr,
+                              const double* pos = nullptr) {
   ipg_ctx* ctx = sn->ctx;
   const SnapDev& P = sn->dev;
   const int64_t n = c.R.n;
@@ -626,6 +667,7 @@ static void energy_forces_dev(ipg_snap* sn, const SnapCall& c, const double* d_b
   double* F = d_forces ? d_forces : scratch_t<double>(ctx, "snap_Fscratch", 3 * std::max<int64_t>(n, 1));
   if (n == 0) {
     zero(ctx, esum, sizeof(double) * 2);
+    if (d_virial) zero(ctx, d_virial, sizeof(double) * 6);
   } else if (path == IPG_SNAP_PLAIN_FP64) {
     double* Z = run_descriptors_fp64(sn, c, nullptr, nullptr, eatom, d_beta, true);
     double* dbl = run_dblist(sn, c, Z);
@@ -634,6 +676,16 @@ static void energy_forces_dev(ipg_snap* sn, const SnapCall& c, const double* d_b
     launch(ctx, "snap_force_from_db",
            [&] { k_force_from_db<<<grid_for(3 * n, 256), 256, 0, ctx->stream>>>(n, M, dB, d_beta, F); });
     if (d_energy) reduce_rows(ctx, eatom, n, 1, d_energy, false, beta0);  // E = beta0 + sum_i E_i
+    if (d_virial) {
+      // the reference's route: W of compute_all_descriptors, then W * beta
+      double* Wa = scratch_t<double>(ctx, "snap_Wa", 6 * (size_t)M * n);
+      double* Wd = scratch_t<double>(ctx, "snap_W", 6 * (size_t)M);
+      launch(ctx, "snap_w_atom", [&] {
+        k_w_atom<<<grid_for(n * (int64_t)M, 256), 256, 0, ctx->stream>>>(P, c.R, pos, dbl, Wa);
+      });
+      reduce_rows(ctx, Wa, n, 6 * M, Wd, false);
+      launch(ctx, "snap_w_beta", [&] { k_w_beta<<<1, 64, 0, ctx->stream>>>(M, Wd, d_beta, d_virial); });
+    }
     return;
   } else {
     double* dedr = scratch_t<double>(ctx, "snap_dedr", 3 * std::max<int64_t>(c.R.npairs, 1));
@@ -657,6 +709,13 @@ static void energy_forces_dev(ipg_snap* sn, const SnapCall& c, const double* d_b
     }
     force_gather(ctx, c.R, dedr, F);
     if (d_energy) reduce_rows(ctx, eatom, n, 1, d_energy, false, beta0);  // E = beta0 + sum_i E_i
+    if (d_virial) {
+      double* va = scratch_t<double>(ctx, "snap_vatom", 6 * (size_t)n);
+      launch(ctx, "snap_virial_atoms", [&] {
+        k_virial_atoms<<<grid_for(n, 256), 256, 0, ctx->stream>>>(n, c.R.off, c.R.nbr, pos, dedr, va);
+      });
+      reduce_rows(ctx, va, n, 6, d_virial, false);
+    }
     return;
   }
   if (d_energy)
@@ -799,6 +858,7 @@ static void snap_energy_forces_call(ipg_snap* sn, const DevSystem& s, int mode, 
     sync(ctx);
     g.replays++;
     if (ctx->prof) add_graph_prof(ctx, g.events);
+    if (over & 16) fail(IPG_ERR_INVALID, "species index out of range of snap.species");
     if (over == 0) return;
     if (over & 11) g.overflows++;   // a list or a neighbor row outgrew its capacity
     keep_graph = (over & 11) == 0;  // only reverse pairs were missing: the capacities hold
@@ -806,11 +866,12 @@ static void snap_energy_forces_call(ipg_snap* sn, const DevSystem& s, int mode, 
   // exact run: the result of this call
   SnapCall c = snap_prepare(sn, s, mode, d_appear, ef_need(path));
   energy_forces_dev(sn, c, d_beta, beta0, path, d_energy, d_forces);
-  if (c.R.rev) {
+  {
     int32_t f = 0;
     d2h(ctx, &f, c.flag, sizeof(int32_t));
     sync(ctx);
-    if (f & 4) fix_asymmetric(sn, c, path, d_beta, d_forces);
+    if (f & 16) fail(IPG_ERR_INVALID, "species index out of range of snap.species");
+    if (c.R.rev && (f & 4)) fix_asymmetric(sn, c, path, d_beta, d_forces);
   }
   if (!use_graph || keep_graph) return;
   graph_release(ctx, g);
@@ -1006,6 +1067,43 @@ int ipg_snap_energy_forces(ipg_snap* sn, const ipg_system* sys, int mode, const 
     if (forces) d2h(ctx, forces, out + 1, sizeof(double) * 3 * s.n);
     sync(ctx);
   });
+}
+
+int ipg_snap_energy_forces_virial(ipg_snap* sn, const ipg_system* sys, int mode, const int32_t* appear,
+                                  const double* beta, int64_t nbeta, double beta0, int path, double* energy,
+                                  double* forces, double* virial) {
+  return guard([&] {
+    IPG_REQUIRE(sn && sys, "NULL argument");
+    IPG_REQUIRE(std::isfinite(beta0), "beta0 must be finite");
+    IPG_REQUIRE(path == IPG_SNAP_PLAIN_FP64 || path == IPG_SNAP_ADJOINT_FP64 || path == IPG_SNAP_ADJOINT_FP32,
+                "unknown SNAP path");
+    ipg_ctx* ctx = sn->ctx;
+    IPG_HIP(hipSetDevice(ctx->device));
+    check_species_host(sn, sys, appear);
+    const double* db = expand_beta(sn, beta, nbeta);
+    DevSystem s = upload_system(ctx, sys, "snap_api");
+    SnapCall c = snap_prepare(sn, s, mode, upload_appear(ctx, appear, s.n), ef_need(path));
+    double* out = scratch_t<double>(ctx, "snap_api_vout", 8 + 3 * std::max<int64_t>(s.n, 1));
+    energy_forces_dev(sn, c, db, beta0, path, out, out + 8, out + 1, s.pos);
+    int32_t f = 0;
+    d2h(ctx, &f, c.flag, sizeof(int32_t));
+    sync(ctx);
+    // the virial sums own rows only; a missing reverse pair changes forces
+    if (c.R.rev && (f & 4)) fix_asymmetric(sn, c, path, db, out + 8);
+    double h[7];
+    d2h(ctx, h, out, sizeof(double) * 7);
+    if (forces) d2h(ctx, forces, out + 8, sizeof(double) * 3 * s.n);
+    sync(ctx);
+    if (energy) *energy = h[0];
+    if (virial)
+      for (int k = 0; k < 6; k++) virial[k] = h[1 + k];
+  });
+}
+
+int ipg_snap_virial_stress(ipg_snap* sn, const ipg_system* sys, int mode, const int32_t* appear, const double* beta,
+                           int64_t nbeta, int path, double* virial) {
+  if (!virial) return guard([] { fail(IPG_ERR_INVALID, "virial is NULL"); });
+  return ipg_snap_energy_forces_virial(sn, sys, mode, appear, beta, nbeta, 0.0, path, nullptr, nullptr, virial);
 }
 
 int ipg_snap_energy_forces_dev(ipg_snap* sn, const ipg_system* sys, int mode, const double* beta, double beta0,

@@ -40,7 +40,7 @@ __global__ __launch_bounds__(64 * ROWS_WAVES) void k_batch_rows(const int64_t* _
   for (int s = 0; s < 8; s++) acc[s] = 0.0;
   if (e < width)
     for (int64_t a = a0 + w; a < a1; a += ROWS_WAVES) {
-      const int sa = species ? species[a] : 0;
+      const int sa = snap_species(species, a, nspecies);
       const double v = rows[a * width + e];
 #pragma unroll
       for (int s = 0; s < 8; s++)
@@ -176,6 +176,7 @@ static SnapListDev batch_run(ipg_snap* sn, const BatchSys& b, int fp32, const Ba
   R.rev = L.rev;
   R.apw = tn.ui_apw;
   R.dbdr_threads = tn.dbdr_threads;
+  R.flag = flag;
   const int64_t n = std::max<int64_t>(b.n, 1);
   double* Bv = o.B ? o.B : scratch_t<double>(ctx, "batch_B", (size_t)n * P.ncoeff);
   if (fp32) {
@@ -355,6 +356,7 @@ static void batch_descriptors(ipg_snap* sn, const BatchSys& b, int precision, co
     sync(ctx);
     g.replays++;
     if (ctx->prof) add_graph_prof(ctx, g.events);
+    if (over & 16) fail(IPG_ERR_INVALID, "species index out of range of snap.species");
     if (over == 0) return;
     g.overflows++;
   }
@@ -366,6 +368,7 @@ static void batch_descriptors(ipg_snap* sn, const BatchSys& b, int precision, co
   int32_t f = 0;
   d2h(ctx, &f, flag, sizeof(int32_t));
   sync(ctx);
+  if (f & 16) fail(IPG_ERR_INVALID, "species index out of range of snap.species");
   if (f & 4) {
     general = true;
     L = batch_run(sn, b, fp32, o, 0, 0, tn, flag, true);

@@ -104,12 +104,12 @@ __global__ __launch_bounds__(DBDR_THREADS, sizeof(S) == 4 ? 3 : 2) void k_snap_d
   for (int e = threadIdx.x; e < 2 * NZ; e += blockDim.x) Zs[e] = zg[e];
   __syncthreads();
   const int64_t q0 = R.off[a], q1 = R.off[a + 1];
-  const int si = R.species ? R.species[a] : 0;
+  const int si = snap_species(R.species, a, P.nspecies);
   for (int64_t it = threadIdx.x; it < 3 * (q1 - q0); it += blockDim.x) {
     const int64_t q = q0 + it / 3;
     const int k = (int)(it % 3);
     const int32_t jn = R.nbr[q];
-    const int sj = R.species ? R.species[jn] : 0;
+    const int sj = snap_species(R.species, jn, P.nspecies);
     typename WideOf<S>::type x, y, z;
     load_pair<typename WideOf<S>::type>(R, q, (int32_t)a, jn, x, y, z);
     const double rcut = (P.radii[si] + P.radii[sj]) * P.rcutfac;

@@ -507,14 +507,15 @@ __global__ __launch_bounds__(64) void k_snap_ui(SnapDev P, SnapRun R, S* __restr
   int nch = (int)((q1 - q0 + G - 1) / G);
   nch = max(max(__builtin_amdgcn_readlane(nch, 0), __builtin_amdgcn_readlane(nch, 16)),
             max(__builtin_amdgcn_readlane(nch, 32), __builtin_amdgcn_readlane(nch, 48)));
-  const int si = valid && R.species ? R.species[a] : 0;
+  if (valid && l == 0 && R.species && R.flag && (unsigned)R.species[a] >= (unsigned)P.nspecies) atomicOr(R.flag, 16);
+  const int si = valid ? snap_species(R.species, a, P.nspecies) : 0;
   for (int ch = 0; ch < nch; ch++) {
     const int64_t q = q0 + (int64_t)ch * G + l;
     const bool act = q < q1;
     CK<S> c;
     if (act) {
       const int32_t j = R.nbr[q];
-      const int sj = R.species ? R.species[j] : 0;
+      const int sj = snap_species(R.species, j, P.nspecies);
       typename WideOf<S>::type x, y, z;
       load_pair<typename WideOf<S>::type>(R, q, (int32_t)a, j, x, y, z);
       const double rcut = (P.radii[si] + P.radii[sj]) * P.rcutfac;
@@ -584,7 +585,7 @@ __global__ __launch_bounds__(64) void k_snap_ui(SnapDev P, SnapRun R, S* __restr
 }
 
 // ---------------------------------------------------------------------------
-// K2: Z (compute_z.jl:1-62), B (compute_b.jl:67-157) and the adjoint
+// K2: Z (compute_z.jl:1-62), B (compute_b.jl:5-95) and the adjoint
 // Y = dE/dU.  Persistent: each 256-thread block stages the Clebsch-Gordan
 // table in LDS once, keeps its Z entries' compact plans in registers and
 // loops over atoms; U and Z of the current atom live in LDS.
@@ -627,7 +628,7 @@ __global__ __launch_bounds__(ZBY_THREADS) void k_snap_zby(SnapDev P, SnapRun R, 
   const int M = P.ncoeff * P.nspecies;
   for (int64_t a = blockIdx.x; a < R.n; a += gridDim.x) {
     __syncthreads();  // the previous atom's readers of U and Z are done
-    const int s = R.species ? R.species[a] : 0;
+    const int s = snap_species(R.species, a, P.nspecies);
     const S* ug = ulisttot + a * (int64_t)P.idxu_max * 2;
     for (int e = tid; e < 2 * P.idxu_max; e += ZBY_THREADS) U[e] = ug[e];
     __syncthreads();
@@ -738,7 +739,7 @@ __global__ __launch_bounds__(256) void k_snap_dedr(SnapDev P, SnapRun R, const S
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= (R.d_np ? *R.d_np : R.npairs)) return;
   const int32_t i = R.center[q], j = R.nbr[q];
-  const int si = R.species ? R.species[i] : 0, sj = R.species ? R.species[j] : 0;
+  const int si = snap_species(R.species, i, P.nspecies), sj = snap_species(R.species, j, P.nspecies);
   typename WideOf<S>::type x, y, z;
   load_pair<typename WideOf<S>::type>(R, q, i, j, x, y, z);
   const double rcut = (P.radii[si] + P.radii[sj]) * P.rcutfac;

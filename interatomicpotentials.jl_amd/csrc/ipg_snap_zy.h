@@ -1,5 +1,5 @@
 // ipg_snap_zy.h -- FP32 adjoint kernel: Z (compute_z.jl:1-62), the energy
-// beta . B (compute_b.jl:67-157, linear_potential.jl:20-26) and the adjoint
+// beta . B (compute_b.jl:5-95, linear_potential.jl:20-26) and the adjoint
 // Y = dE/dU for batches of 32 atoms, with every lane owning one atom.
 //
 // Work layout.  The output rows (j, mb), mb <= j/2, of Y are the units of
@@ -387,7 +387,7 @@ __global__ __launch_bounds__(64 * NW) void k_snap_zy32(SnapDev P, int64_t n, con
     }
   }
   __syncthreads();
-  const int s = valid && species ? species[a] : 0;
+  const int s = valid ? snap_species(species, a, P.nspecies) : 0;
   const double* ycs = ycL + s * P.nblk;
   const double* bts = btL + s * P.ncoeff;
   float2* yb = DESC ? nullptr : Yb + batch * (int64_t)IDXU * zy::AB + at;

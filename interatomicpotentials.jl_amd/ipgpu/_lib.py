@@ -33,7 +33,8 @@ EXPORTED = [
     "ipg_snap_force_descriptors", "ipg_snap_all_descriptors", "ipg_snap_energy_forces",
     "ipg_snap_energy_forces_dev", "ipg_snap_force_vjp", "ipg_ctx_set_graphs", "ipg_snap_graph_info",
     "ipg_device_copy", "ipg_snap_batch_descriptors", "ipg_snap_batch_descriptors_dev", "ipg_snap_batch_info",
-    "ipg_nlist_build_skin_dev", "ipg_nlist_update_dev", "ipg_prof_select",
+    "ipg_nlist_build_skin_dev", "ipg_nlist_update_dev", "ipg_prof_select", "ipg_snap_virial_stress",
+    "ipg_snap_energy_forces_virial",
 ]
 
 
@@ -126,6 +127,8 @@ def load():
             "ipg_snap_all_descriptors": (C.c_int, [P, pS, C.c_int, P, P, P, P]),
             "ipg_snap_energy_forces": (C.c_int, [P, pS, C.c_int, P, P, i64, d, C.c_int, P, P]),
             "ipg_snap_energy_forces_dev": (C.c_int, [P, pS, C.c_int, P, d, C.c_int, P, P]),
+            "ipg_snap_virial_stress": (C.c_int, [P, pS, C.c_int, P, P, i64, C.c_int, P]),
+            "ipg_snap_energy_forces_virial": (C.c_int, [P, pS, C.c_int, P, P, i64, d, C.c_int, P, P, P]),
             "ipg_snap_force_vjp": (C.c_int, [P, pS, C.c_int, P, P, i64, P, C.c_int, P, P]),
             "ipg_ctx_set_graphs": (C.c_int, [P, C.c_int]),
             "ipg_device_copy": (C.c_int, [P, P, P, C.c_size_t]),

@@ -9,7 +9,7 @@ add and multiply once (a complex multiply is 6 flops):
     16 flops.  ulisttot accumulation: 4 flops per emitted entry.
   * Z (compute_z.jl): 10 flops per inner term, 4 per outer term; B: 4 per
     half-sum term; Y: 4 per contributing Z entry.
-  * Plain dB (compute_b.jl:166-404): 4 flops per half-sum term per component,
+  * Plain dB (compute_b.jl:104-342): 4 flops per half-sum term per component,
     plus the switching-function scaling (24 per left-half entry).
 Bytes for the pair kernel (HBM-bound): per step N(3w + 4) positions+species
 read, 3Nw forces written, 4 bytes per neighbor index read, 56 bytes of

@@ -266,6 +266,43 @@ def energy_and_force(s, lbp, path="adjoint", list_mode=None):
     return EnergyForce(e.value, f)
 
 
+def virial_stress(s, lbp, path="adjoint", list_mode=None):
+    """basis_potentials.jl:43-50 with linear_potential.jl:37-42: W * beta,
+    W the virial descriptors of compute_all_descriptors (snap.jl:267-281,
+    raw positions), rows [xx, yy, zz, zy, zx, yx].  path 'plain' forms W
+    from the per-pair dB as the reference does; the adjoint paths contract
+    the per-pair dE/dr with x_i - x_j on the device."""
+    L = _lib.load()
+    snap = lbp.basis
+    cs, keep, appear = snap._sys(s)
+    v = np.zeros(6)
+    check(L.ipg_snap_virial_stress(snap._h, C.byref(cs), snap._mode(list_mode), appear.ctypes.data,
+                                   lbp.beta.ctypes.data, lbp.beta.size, _PATHS[path], v.ctypes.data))
+    return v
+
+
+def virial(s, lbp, **kw):
+    """basis_potentials.jl:52-57: the sum of the six virial_stress entries."""
+    return float(np.sum(virial_stress(s, lbp, **kw)))
+
+
+EnergyForceVirial = collections.namedtuple("EnergyForceVirial", ["e", "f", "v"])
+
+
+def energy_force_virial(s, lbp, path="adjoint", list_mode=None):
+    """Energy, forces and virial stress from one evaluation."""
+    L = _lib.load()
+    snap = lbp.basis
+    cs, keep, appear = snap._sys(s)
+    e = C.c_double()
+    f = np.zeros((cs.natoms, 3))
+    v = np.zeros(6)
+    check(L.ipg_snap_energy_forces_virial(snap._h, C.byref(cs), snap._mode(list_mode), appear.ctypes.data,
+                                          lbp.beta.ctypes.data, lbp.beta.size, lbp.beta0, _PATHS[path],
+                                          C.byref(e), f.ctypes.data, v.ctypes.data))
+    return EnergyForceVirial(e.value, f, v)
+
+
 def potential_energy(s, lbp, **kw):
     return energy_and_force(s, lbp, **kw).e
 

@@ -182,7 +182,9 @@ function InteratomicPotentials.virial_stress(s::AbstractSystem, h::HIP{<:Union{E
     _, _, v = pair_eval(s, h.p)
     SVector{6}(v) * u"hartree"
 end
-InteratomicPotentials.virial(s::AbstractSystem, h::HIP) = sum(virial_stress(s, h)[1:3])
+# abstract_potential.jl:10: the trace of the virial stress
+InteratomicPotentials.virial(s::AbstractSystem, h::HIP{<:Union{EmpiricalPotential,LinearCombinationPotential}}) =
+    sum(virial_stress(s, h)[1:3])
 
 # -- SNAP -------------------------------------------------------------------
 mutable struct SnapHandle
@@ -278,6 +280,26 @@ function InteratomicPotentials.energy_and_force(s::AbstractSystem, h::HIP{<:LBas
     e, f = snap_energy_forces(s, h.p, path)
     (; e, f)
 end
+
+# basis_potentials.jl:43-50 with linear_potential.jl:37-42: SVector{6}(W * β),
+# W the virial descriptors of compute_all_descriptors (snap.jl:267-281).
+function InteratomicPotentials.virial_stress(s::AbstractSystem, h::HIP{<:LBasisPotential}; path=IPG_SNAP_ADJOINT_FP64,
+                                             mode=IPG_SNAP_LIST_REFERENCE)
+    lbp = h.p
+    snap = lbp.basis
+    H = handle(snap)
+    appear = snap_appear(s)
+    β = Vector{Float64}(lbp.β)
+    with_system(s, ustrip, snap_species(s, snap)) do sys, X
+        v = zeros(Float64, 6)
+        check(ccall((:ipg_snap_virial_stress, libipgpu), Cint,
+                    (Ptr{Cvoid}, Ref{IPGSystem}, Cint, Ptr{Int32}, Ptr{Float64}, Int64, Cint, Ptr{Float64}),
+                    H.ptr, sys, mode, appear, β, length(β), path, v))
+        SVector{6}(v)
+    end
+end
+# basis_potentials.jl:52-57 and linear_potential.jl:44-49: the sum of all six entries
+InteratomicPotentials.virial(s::AbstractSystem, h::HIP{<:LBasisPotential}; kw...) = sum(virial_stress(s, h; kw...))
 
 """
     rrule(force, s, h::HIP{<:LBasisPotential})

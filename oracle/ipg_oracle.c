@@ -210,24 +210,24 @@ int64_t orc_nlist_cells(const double* pos, int64_t n, const double L[3], double 
 /* pair potentials                                                    */
 /* ================================================================== */
 
-/* src/empirical/zbl.jl:25-26 */
+/* src/empirical/zbl.jl:27-28 */
 static const double ZBL_C[4] = {0.18175, 0.50986, 0.28022, 0.02817};
 static const double ZBL_X[4] = {3.19980, 0.94229, 0.40290, 0.20162};
 
-static double zbl_phi(double d, double e) { /* zbl.jl:27 */
+static double zbl_phi(double d, double e) { /* zbl.jl:29 */
   double s = ZBL_C[0] * pow(e, -ZBL_X[0] * d);
   for (int k = 1; k < 4; k++) s = s + ZBL_C[k] * pow(e, -ZBL_X[k] * d);
   return s;
 }
-static double zbl_dphi(double d, double e) { /* zbl.jl:28 */
+static double zbl_dphi(double d, double e) { /* zbl.jl:30 */
   double s = ZBL_X[0] * ZBL_C[0] * pow(e, -ZBL_X[0] * d);
   for (int k = 1; k < 4; k++) s = s + ZBL_X[k] * ZBL_C[k] * pow(e, -ZBL_X[k] * d);
   return -s;
 }
-static double zbl_coeff(const orc_pair_term* t) { /* zbl.jl:30, k_e = 1 (constants.jl:7) */
+static double zbl_coeff(const orc_pair_term* t) { /* zbl.jl:32, k_e = 1 (constants.jl:7) */
   return 1.0 * t->p[0] * t->p[1] * (t->p[2] * t->p[2]);
 }
-static double zbl_d(const orc_pair_term* t, double R) { /* zbl.jl:31 */
+static double zbl_d(const orc_pair_term* t, double R) { /* zbl.jl:33 */
   return R / (0.8854 * 0.529 / (pow(t->p[0], 0.23) + pow(t->p[1], 0.23)));
 }
 
@@ -243,7 +243,7 @@ double orc_pair_V(const orc_pair_term* t, double R) {
       double q = 1.0 - E;
       return p[0] * (q * q);
     }
-    case ORC_PAIR_ZBL: /* zbl.jl:33 */
+    case ORC_PAIR_ZBL: /* zbl.jl:35 */
       return zbl_coeff(t) * zbl_phi(zbl_d(t, R), p[2]) / R;
     case ORC_PAIR_BM: /* bm.jl:38 */
       return p[0] * exp((p[2] - R) / p[1]) + pow(p[3] / R, 6) - pow(p[4] / R, 8);
@@ -269,7 +269,7 @@ void orc_pair_F(const orc_pair_term* t, double R, const double r[3], double out[
       s = 2.0 * p[0] * p[1] * (1.0 - E) * E / R;
       break;
     }
-    case ORC_PAIR_ZBL: { /* zbl.jl:34 */
+    case ORC_PAIR_ZBL: { /* zbl.jl:36 */
       double d = zbl_d(t, R);
       s = zbl_coeff(t) * (zbl_phi(d, p[2]) + d * zbl_dphi(d, p[2])) / (R * R * R);
       break;
@@ -747,7 +747,7 @@ static void compute_zi(const orc_snap_params* p, orc_rt* rt) {
     }
 }
 
-/* compute_b.jl:67-157 compute_bi */
+/* compute_b.jl:5-95 compute_bi (half-column sums :36-62, bzero shift :70-93) */
 static void compute_bi(const orc_snap_params* p, orc_rt* rt) {
   const orc_snap_tables* t = rt->t;
   int nel = t->num_elements;
@@ -896,7 +896,7 @@ static void compute_duidrj(const orc_snap_params* p, const orc_snap_tables* t, c
 }
 
 /* half-column sum of Re(conj(du_k) * z) over level jl, as in
- * compute_b.jl:225-266 (and the j1, j2 variants :285-326, :351-391). */
+ * compute_b.jl:150-208 (and the j1, j2 variants :210-272, :274-337). */
 static void half_sum_du_z(const orc_snap_tables* t, int jl, int jju, int jjz, const double* dr, const double* di,
                           const double* zr, const double* zi, double out[3]) {
   out[0] = out[1] = out[2] = 0.0;
@@ -918,7 +918,7 @@ static void half_sum_du_z(const orc_snap_tables* t, int jl, int jju, int jjz, co
   (void)t;
 }
 
-/* compute_b.jl:166-404 compute_dbidrj; dblist layout [(itriple*idxb_max+jjb)*3 + k] */
+/* compute_b.jl:104-342 compute_dbidrj; dblist layout [(itriple*idxb_max+jjb)*3 + k] */
 static void compute_dbidrj(const orc_snap_params* p, orc_rt* rt, int jelem) {
   const orc_snap_tables* t = rt->t;
   int nel = t->num_elements;

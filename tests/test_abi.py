@@ -86,3 +86,28 @@ def test_default_context_returns_promptly():
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     assert r.stdout.startswith("ctx") or r.stdout.startswith("err -3")
+
+
+def test_struct_layouts_match_the_header():
+    """The ctypes mirrors (and the Julia structs, which use the same natural
+    alignment) match the sizes and offsets the library asserts at compile
+    time (static_assert in csrc/ipg_core.hip)."""
+    from ipgpu import _lib
+
+    want = {
+        _lib.System: (64, {"natoms": 0, "positions": 8, "species": 16, "box": 24, "periodic": 48}),
+        _lib.PairTerm: (80, {"kind": 0, "reserved": 4, "coeff": 8, "rcut": 16, "p": 24, "species_mask": 72}),
+        _lib.SnapParams: (72, {"twojmax": 0, "nspecies": 4, "rcutfac": 8, "rmin0": 16, "rfac0": 24, "radii": 32,
+                               "weight": 40, "chem_flag": 48, "bzero_flag": 52, "bnorm_flag": 56,
+                               "switch_flag": 60, "wselfall_flag": 64}),
+    }
+    for cls, (size, offs) in want.items():
+        assert C.sizeof(cls) == size, cls
+        for name, off in offs.items():
+            assert getattr(cls, name).offset == off, (cls, name)
+    src = open(os.path.join(REPO, "interatomicpotentials.jl_amd", "csrc", "ipg_core.hip")).read()
+    for cls, (size, offs) in want.items():
+        cname = {"System": "ipg_system", "PairTerm": "ipg_pair_term", "SnapParams": "ipg_snap_params"}[cls.__name__]
+        assert "sizeof(%s) == %d" % (cname, size) in src
+        for name, off in offs.items():
+            assert "offsetof(%s, %s) == %d" % (cname, name, off) in src

@@ -162,3 +162,77 @@ def test_batch_512_configs_properties(ipg):
     scale = np.abs(dB64).max()
     for c in range(0, nconf, 37):
         assert np.abs(dB64[start[c]:start[c + 1]].sum(0)).max() <= 1e-9 * scale * (start[c + 1] - start[c])
+
+
+def test_snap_2J8_8192_atoms_vs_oracle(ipg, orc):
+    """The energy/force bench workload against the oracle at full size: the
+    CUTOFF list bit for bit (the three-launch binning path), FP64 plain and
+    adjoint energy and forces within 1e-10 relative of the oracle's plain
+    route (OpenMP, 16 threads at most)."""
+    import os
+
+    from ipgpu import workloads
+
+    s = workloads.snap_bcc(ncell=16, a=3.16, jitter=0.05, nspecies=2, seed=5)
+    names = ["W", "Mo"]
+    sp = np.array([names.index(x) for x in s.symbols], np.int32)
+    sn = ipg.snap.SNAP(len(s), 8, names, 5.0, 0.0, 0.99363, [0.5, 0.5], [1.0, 0.9], switch_flag=True,
+                       list_mode="cutoff")
+    o = orc.Snap(8, 2, 5.0, 0.0, 0.99363, [0.5, 0.5], [1.0, 0.9], switch_flag=True)
+    off, nbr, R, r = o.neighborlist(s.positions, s.box, sp, mode=orc.SNAP_LIST_CUTOFF)
+    nl = ipg.snap.snap_neighborlist(s, sn)
+    np.testing.assert_array_equal(nl.offsets, off)
+    np.testing.assert_array_equal(nl.j, nbr)
+    np.testing.assert_array_equal(nl.r, r)
+    beta = np.random.default_rng(55).normal(scale=0.1, size=2 * sn.ncoeff)
+    eo, fo, _ = o.plain_energy_forces(sp, off, nbr, r, beta, nthreads=min(16, os.cpu_count() or 1))
+    lbp = ipg.snap.LBasisPotential(beta, 0.0, sn)
+    for path in ("plain", "adjoint"):
+        e, f = ipg.snap.energy_and_force(s, lbp, path=path)
+        assert abs(e - eo) <= 1e-10 * abs(eo), path
+        assert rel(f, fo) < 1e-10, path
+
+
+def test_mixed_pairs_256k_vs_oracle(ipg, orc):
+    """The 256,000-atom LJ + Morse + ZBL workload: FP64 energy, forces and
+    virial within 1e-12 relative of the oracle, which evaluates the three
+    potentials one by one with a cell list each, as
+    LinearCombinationPotential does."""
+    from test_gpu_pair import close, oracle_eval
+
+    from ipgpu import workloads
+
+    s, pot = workloads.mixed_pairs()
+    e, f = ipg.energy_and_force(s, pot)
+    vir = ipg.virial_stress(s, pot)
+    eo, fo, vo = oracle_eval(orc, s, pot)
+    close(e, eo)
+    close(f, fo)
+    close(vir, vo)
+
+
+def test_batch_512_configs_vs_oracle(ipg, orc):
+    """Four of the 512 batch configurations (the largest, the smallest and
+    two others) against the oracle's compute_all_descriptors: FP64 batch
+    within 1e-10 relative, FP32 batch within 2e-5."""
+    from ipgpu import workloads
+
+    names = ["W", "Mo"]
+    systems = workloads.snap_batch(nconfig=512, nmin=50, nmax=2000, rcut=5.0, seed=4)
+    sn = ipg.snap.SNAP(100, 8, names, 5.0, 0.0, 0.99363, [0.5, 0.5], [1.0, 0.9], switch_flag=True,
+                       list_mode="cutoff")
+    o = orc.Snap(8, 2, 5.0, 0.0, 0.99363, [0.5, 0.5], [1.0, 0.9], switch_flag=True)
+    sizes = np.array([len(x) for x in systems])
+    pick = [int(np.argmax(sizes)), int(np.argmin(sizes)), 100, 400]
+    r64 = ipg.snap.compute_batch_descriptors(systems, sn, precision="fp64")
+    r32 = ipg.snap.compute_batch_descriptors(systems, sn, precision="fp32")
+    for c in pick:
+        sc = systems[c]
+        sp = np.array([names.index(x) for x in sc.symbols], np.int32)
+        off, nbr, R, rv = o.neighborlist(sc.positions, sc.box, sp, mode=orc.SNAP_LIST_CUTOFF)
+        Bo, dBo, Wo = o.all_descriptors(sc.positions, sp, off, nbr, rv)
+        a, b = r64.start[c], r64.start[c + 1]
+        assert rel(r64.B[a:b], Bo) < 1e-10, c
+        assert rel(r64.dB[a:b], dBo.transpose(0, 2, 1)) < 1e-10, c
+        assert rel(r64.W[c], Wo.T) < 1e-10, c
+        assert rel(r32.B[a:b], Bo) < 2e-5 and rel(r32.dB[a:b], dBo.transpose(0, 2, 1)) < 2e-5, c

@@ -387,3 +387,153 @@ def test_plain_energy_forces_equals_descriptors(orc, nthreads):
     off_k = np.minimum(off, off[k])
     ad = s.adjoint_energy_forces(sp, off_k, nb, r, beta)
     assert np.abs(F_k - ad["forces"]).max() < 1e-12 * np.abs(ad["forces"]).max()
+
+
+# ------------------------------------------------- SNAP list (as-is) ----
+def snap_list_fixture():
+    with open(os.path.join(GOLDEN, "snap_list_hand.json")) as f:
+        return json.load(f)
+
+
+def first_appearance(symbols):
+    seen = []
+    for s in symbols:
+        if s not in seen:
+            seen.append(s)
+    return np.array([seen.index(s) for s in symbols], dtype=np.int32)
+
+
+@pytest.mark.parametrize("mode", ["reference", "cutoff"])
+def test_snap_list_hand_derived(orc, mode):
+    """utl.jl:38-101 on a 6-atom, two-species, mixed-boundary system whose
+    lists were worked out by hand (tests/golden/snap_list_hand.json): the
+    sqrt(cutmax) radius, the first-appearance element order and the
+    enumerate-index element lookup each change at least one row."""
+    v = snap_list_fixture()
+    X = np.array(v["positions"])
+    sp = np.array([v["snap_species"].index(s) for s in v["symbols"]], dtype=np.int32)
+    ap = first_appearance(v["symbols"])
+    L = [v["box"][d] if v["periodic"][d] else INF for d in range(3)]
+    s = orc.Snap(2, 2, v["rcutfac"], 0.0, 0.99363, v["radii"], [1.0, 1.0])
+    m = orc.SNAP_LIST_REFERENCE if mode == "reference" else orc.SNAP_LIST_CUTOFF
+    off, j, R, r = s.neighborlist(X, L, sp, appear=ap, mode=m)
+    want = v[mode]
+    assert [list(j[off[i]:off[i + 1]] + 1) for i in range(len(X))] == want["j_1based"]
+    rw = np.array([x for row in want["r"] for x in row]).reshape(-1, 3)
+    np.testing.assert_array_equal(r, rw)
+    np.testing.assert_array_equal(R, np.sqrt([x for row in want["rsq"] for x in row]))
+
+
+# ----------------------------------------------------------- bzero ----
+@pytest.mark.parametrize("J", [2, 5, 8])
+def test_bzero_isolated_atom_is_zero(orc, J):
+    """bzero (prebuilt_arrays.jl:52-63, compute_b.jl:70-93) is the bispectrum
+    of an isolated atom: its U_j are identity matrices (w_self = 1 on the
+    diagonal, compute_ui.jl:131-152), so Z_j = I by CG orthogonality and
+    B(j1, j2, j) = tr I = j + 1.  With bzero_flag every entry of an isolated
+    atom's B is therefore 0; a shift read at j1 or j2 instead of j, or a
+    dropped (j+1), leaves entries nonzero.  Without the flag B = j + 1."""
+    s0 = orc.Snap(J, 1, 2.0, 0.0, 0.99363, [1.0], [1.0])
+    s1 = orc.Snap(J, 1, 2.0, 0.0, 0.99363, [1.0], [1.0], bzero_flag=True)
+    n = 3
+    off = np.zeros(n + 1, dtype=np.int64)
+    sp = np.zeros(n, dtype=np.int32)
+    nb, rv = np.zeros(0, np.int32), np.zeros((0, 3))
+    assert np.abs(s1.local_descriptors(sp, off, nb, rv)).max() < 1e-13
+    B0 = s0.local_descriptors(sp, off, nb, rv)
+    np.testing.assert_allclose(B0, np.broadcast_to(s0.idxb()[:, 2] + 1.0, B0.shape), rtol=1e-14)
+
+
+@pytest.mark.parametrize("J", [2, 5, 8])
+@pytest.mark.parametrize("bzero", [False, True])
+def test_chem_isolated_atoms_as_is(orc, J, bzero):
+    """chem_flag = true (bnorm forced on) for isolated atoms of a two-element
+    basis, worked out by hand from the as-is loops.  Only the own element's
+    U is the identity (zero_uarraytot, compute_ui.jl:131-152).
+    compute_zi (compute_z.jl:5-61) clears slot idouble*idxz_max + jjz but
+    accumulates into slot jjz and divides it by j+1 after every element
+    pair (:46-56), so only Z block 0 is ever nonzero: for element 0 it ends
+    at I/(j+1)^4 (pairs (0,0), (0,1), (1,0), (1,1) each divide once), for
+    element 1 at I/(j+1) (only pair (1,1) adds).  compute_bi (compute_b.jl:
+    17-93) pairs Z block (e1, e2) with U of element e3, so the nonzero
+    triple is (0, 0, e) with B = tr(Z) = 1/(j+1)^3 (element 0) or 1
+    (element 1).  Its outer ielem loop recomputes every triple and then
+    shifts triple (ielem-1)(nel+1)nel + (ielem-1), so only the last shift,
+    triple (1, 1, 1) = 7, survives: B = -bzero[j] = -1 there (bnorm)."""
+    s = orc.Snap(J, 2, 2.0, 0.0, 0.99363, [1.0, 1.0], [1.0, 1.0], chem_flag=True, bnorm_flag=True,
+                 bzero_flag=bzero)
+    nb_ = s.idxb_max
+    assert s.ncoeff == 8 * nb_
+    sp = np.array([0, 1], dtype=np.int32)
+    B = s.local_descriptors(sp, np.zeros(3, dtype=np.int64), np.zeros(0, np.int32), np.zeros((0, 3)))
+    j = s.idxb()[:, 2].astype(float)
+    want = np.zeros((2, 8, nb_))
+    want[0, 0] = 1.0 / (j + 1) ** 3
+    want[1, 1] = 1.0
+    if bzero:
+        want[:, 7] -= 1.0
+    np.testing.assert_allclose(B, want.reshape(2, -1), rtol=1e-13, atol=1e-15)
+
+
+def test_bzero_shift_with_neighbors(orc):
+    """With neighbors, B(bzero) + bzero[j] equals B without the flag
+    entry by entry (the shift is a subtraction at the triple's j only)."""
+    s0, X, L, sp = _bcc(orc, 6, True, J=8)
+    s1 = orc.Snap(8, 2, 2.5, 0.0, 0.99363, [1.0, 1.0], [1.0, 0.8], switch_flag=True, bzero_flag=True)
+    off, nb, R, r = s0.neighborlist(X, L, sp, mode=orc.SNAP_LIST_CUTOFF)
+    B0 = s0.local_descriptors(sp, off, nb, r)
+    B1 = s1.local_descriptors(sp, off, nb, r)
+    jb = s0.idxb()[:, 2]
+    np.testing.assert_allclose(B1 + (jb + 1.0), B0, rtol=1e-13, atol=1e-13 * np.abs(B0).max())
+
+
+# ------------------------------------------- Born-Mayer and ZBL e != 1 ----
+def test_born_mayer_pins(orc):
+    """bm.jl:38-39.  The exponential term's force is the gradient force
+    (F.R/R = -dV/dR, checked by central differences with C = D = 0); the
+    power terms keep the reference's non-gradient form F = V_pow / R^2 r,
+    checked at a point where every value is exact: C = 2, D = 1, R = 2
+    gives (C/R)^6 = 1, (D/R)^8 = 1/256."""
+    t = dict(kind=orc.PAIR_BM, rcut=10, p=[0.7, 0.3, 1.1, 0.0, 0.0])
+    for R in (0.9, 1.4, 2.6):
+        dV, F = _fd_force(orc, t, R)
+        assert F == pytest.approx(-dV, rel=1e-7)
+    t = dict(kind=orc.PAIR_BM, rcut=10, p=[0.0, 0.3, 1.1, 2.0, 1.0])
+    assert orc.pair_V(t, 2.0) == 1.0 - 1.0 / 256.0
+    np.testing.assert_array_equal(orc.pair_F(t, 2.0, [2.0, 0.0, 0.0]), [(1.0 - 1.0 / 256.0) / 4.0 * 2.0, 0, 0])
+    # both parts together: V and F add (a dropped term fails this)
+    tA = dict(kind=orc.PAIR_BM, rcut=10, p=[0.7, 0.3, 1.1, 0.0, 0.0])
+    tB = dict(kind=orc.PAIR_BM, rcut=10, p=[0.7, 0.3, 1.1, 2.0, 1.0])
+    assert orc.pair_V(tB, 2.0) == pytest.approx(orc.pair_V(tA, 2.0) + 1.0 - 1.0 / 256.0, rel=1e-15)
+
+
+ZBL_TEXTBOOK_C = (0.18175, 0.50986, 0.28022, 0.02817)
+ZBL_TEXTBOOK_X = (3.19980, 0.94229, 0.40290, 0.20162)
+
+
+def zbl_universal_screening(x):
+    """The ZBL universal screening function (Ziegler, Biersack, Littmark,
+    The Stopping and Range of Ions in Solids, 1985)."""
+    return sum(c * np.exp(-k * x) for c, k in zip(ZBL_TEXTBOOK_C, ZBL_TEXTBOOK_X))
+
+
+@pytest.mark.parametrize("e", [np.e, 2.0, 1.7])
+def test_zbl_charge_base(orc, e):
+    """zbl.jl:27-36 with e != 1: e^(-x d) = exp(-x d ln e), so
+    phi(d, e) = Phi(d ln e) with Phi the textbook screening function, and
+    dphi(d, e) = Phi'(d ln e); at e = Euler's number these are the
+    textbook sums.  V = k Z1 Z2 e^2 Phi(d ln e) / R and the as-is force
+    F.R/R = k Z1 Z2 e^2 (Phi + d Phi') / R^2, with Phi' from central
+    differences of Phi (not from the oracle's dphi)."""
+    Z1, Z2 = 18, 54
+    t = dict(kind=orc.PAIR_ZBL, rcut=10, p=[Z1, Z2, e])
+    a = 0.8854 * 0.529 / (Z1 ** 0.23 + Z2 ** 0.23)
+    cf = Z1 * Z2 * e * e
+    for R in (0.05, 0.3, 1.2):
+        d = R / a
+        x = d * np.log(e)
+        assert orc.pair_V(t, R) == pytest.approx(cf * zbl_universal_screening(x) / R, rel=1e-12)
+        h = 1e-6 * max(x, 1e-3)
+        dPhi = (zbl_universal_screening(x + h) - zbl_universal_screening(x - h)) / (2 * h)
+        F = orc.pair_F(t, R, [R, 0.0, 0.0])[0]  # r = R e_x, so F_x = F.r/R
+        assert F == pytest.approx(cf * (zbl_universal_screening(x) + d * dPhi) / R ** 2, rel=1e-7)
