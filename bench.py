@@ -91,22 +91,112 @@ def kernel_avg(names):
     return out
 
 
-def cpu_baseline_snap(s, beta, sp, sample1):
-    """The oracle (C restatement of the reference's plain SNAP route: per
-    atom B, per pair dB/dr contracted with beta, forces scattered to both
-    atoms), timed on this host: all threads (OpenMP, at most 16) over the
-    whole workload, and one thread over the first `sample1` atoms."""
+CPU_RUNS = 10  # timed runs per CPU figure (after one untimed warm-up); the median is reported
+
+
+def host_info():
+    """The host the CPU baselines ran on: logical CPUs, the CPUs this process
+    may run on, the OpenMP thread count used for "all cores" and the CPU
+    model (lscpu's "Model name", read from /proc/cpuinfo)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.lower().startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count() or 1
+    threads = int(os.environ.get("OMP_NUM_THREADS", avail) or avail)
+    threads = max(1, min(threads, avail))
+    return {"nproc": os.cpu_count(), "cpus_available": avail, "threads_all_cores": threads, "model": model}
+
+
+def median_time(fn, runs=CPU_RUNS):
+    """Median wall time of `runs` calls of fn after one untimed warm-up."""
+    fn()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), len(ts)
+
+
+def oracle_module():
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle as O
 
+    return O
+
+
+def cpu_baseline_snap(s, beta, sp, sample1, hi):
+    """The oracle (C restatement of the reference's plain SNAP route: per
+    atom B, per pair dB/dr contracted with beta, forces scattered to both
+    atoms), timed on this host: all cores (OpenMP) over the whole workload,
+    and one thread over the first `sample1` atoms; median of CPU_RUNS."""
+    O = oracle_module()
     o = O.Snap(8, 2, 5.0, 0.0, 0.99363, [0.5, 0.5], [1.0, 0.9], switch_flag=True)
     off, nbr, R, r = o.neighborlist(s.positions, s.box, sp, mode=O.SNAP_LIST_CUTOFF)
     n = len(s)
-    nth = max(1, min(16, os.cpu_count() or 1))
-    _, _, tn = o.plain_energy_forces(sp, off, nbr, r, beta, nthreads=nth)
+    nth = hi["threads_all_cores"]
+    tn, runs = median_time(lambda: o.plain_energy_forces(sp, off, nbr, r, beta, nthreads=nth))
     k = min(sample1, n)
-    _, _, t1 = o.plain_energy_forces(sp, off, nbr, r, beta, nthreads=1, count=k)
-    return {"threads": nth, "rate": n / tn, "seconds": tn, "rate1": k / t1, "seconds1": t1, "sample1": k}
+    t1, _ = median_time(lambda: o.plain_energy_forces(sp, off, nbr, r, beta, nthreads=1, count=k))
+    return {"threads": nth, "rate": n / tn, "seconds": tn, "rate1": k / t1, "seconds1": t1, "sample1": k,
+            "runs": runs}
+
+
+def cpu_baseline_pairs(hi):
+    """The oracle's pair path (every potential of the sum with its own cell
+    list, as LinearCombinationPotential evaluates them; OpenMP over atoms
+    with per-thread sums, as the reference threads _perform_pairwise) on the
+    LJ/Morse/ZBL workload at its density: a 64,000-atom box on all cores and
+    an 8,000-atom box on one thread; and the two 1000-atom LJ workloads in
+    full.  Rates are half pairs per second; median of CPU_RUNS each."""
+    import ipgpu
+    from ipgpu import workloads
+    from ipgpu.empirical import _terms_of
+
+    O = oracle_module()
+
+    def case(s, pot):
+        table = s.unique_symbols()
+        sp = np.array([table.index(x) for x in s.symbols], np.int32)
+        L = [s.austrip_box()[d] if s.periodic[d] else np.inf for d in range(3)]
+        mask = lambda p: O.ALL_SPECIES if p.species is None else sum(1 << k for k, x in enumerate(table)
+                                                                      if x in p.species)
+        terms = [dict(kind=p.kind, coeff=c, rcut=p.rcutoff, p=list(p.params()), species_mask=mask(p))
+                 for p, c in _terms_of(pot)]
+        X = s.austrip_positions()
+        nl = ipgpu.nnlist.neighborlist_arrays(X, s.austrip_box(), s.periodic, pot.get_rcutoff(), want_disp=False)
+        return X, L, sp, terms, int(len(nl[1]))
+
+    out = {}
+    nth = hi["threads_all_cores"]
+    for label, n_side, threads in (("mixed_lj_morse_zbl_64000_all_cores", (40, 40, 40), nth),
+                                   ("mixed_lj_morse_zbl_8000_1_thread", (20, 20, 20), 1)):
+        s, pot = workloads.mixed_pairs(n_side=n_side)
+        X, L, sp, terms, half = case(s, pot)
+        t, runs = median_time(lambda: O.pair_energy_force_virial(X, L, sp, terms, cells=True, nthreads=threads))
+        out[label] = {"natoms": len(s), "half_pairs": half, "threads": threads, "seconds": t,
+                      "pairs_per_s": half / t, "runs": runs}
+    s, pot = workloads.lj_argon_fcc()
+    cl_X = np.loadtxt(os.path.join(REPO, "tests", "golden", "lj_1000.xyz"))
+    cl = ipgpu.AtomsSystem(cl_X, [8.0, 8.0, 8.0], (False,) * 3, ["Ar"] * len(cl_X), unit="angstrom")
+    cl_pot = ipgpu.LennardJones(1.0 * ipgpu.eV, 1.0 * ipgpu.angstrom, 100.0 * ipgpu.angstrom, ["Ar"])
+    for label, (sx, px) in (("lj_argon_fcc_1000", (s, pot)), ("lj_cluster_1000_rcut100", (cl, cl_pot))):
+        X, L, sp, terms, half = case(sx, px)
+        row = {"natoms": len(sx), "half_pairs": half}
+        for tag, threads in (("all_cores", nth), ("1_thread", 1)):
+            t, runs = median_time(lambda: O.pair_energy_force_virial(X, L, sp, terms, cells=True, nthreads=threads))
+            row[tag] = {"threads": threads, "seconds": t, "pairs_per_s": half / t, "runs": runs}
+        out[label] = row
+    return out
 
 
 def batch_bench(steps, warmup, nconfig, cpu_configs):
@@ -158,7 +248,13 @@ def batch_bench(steps, warmup, nconfig, cpu_configs):
             dB32 = dB.download()
     B64, er64, dB64 = B.download(), er.download(), dB.download()
     rel = lambda a, b: float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
-    res["fp32_vs_fp64"] = {"B": rel(B32, B64), "dB": rel(dB32, dB64), "erow": rel(er32, er64)}
+    rms = lambda a, b: float(np.sqrt(np.mean((a - b) ** 2)))
+    res["fp32_vs_fp64"] = {"B": rel(B32, B64), "dB": rel(dB32, dB64), "erow": rel(er32, er64),
+                           "B_rms": rms(B32, B64), "dB_rms": rms(dB32, dB64), "erow_rms": rms(er32, er64),
+                           "B_scale_rms": float(np.sqrt(np.mean(B64 ** 2))),
+                           "dB_scale_rms": float(np.sqrt(np.mean(dB64 ** 2))),
+                           "note": "B, dB, erow: max |fp32 - fp64| / max |fp64|; *_rms: RMS of fp32 - fp64; "
+                                   "*_scale_rms: RMS of the fp64 values"}
     del dB32, dB64
     # per-kernel times of the FP32 pipeline, launched kernel by kernel
     set_graphs(False)
@@ -182,21 +278,20 @@ def batch_bench(steps, warmup, nconfig, cpu_configs):
                                   "tflops": fl / (ms * 1e-3) / 1e12,
                                   "frac_of_fp32_peak": fl / (ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS}
     if cpu_configs > 0:
-        sys.path.insert(0, os.path.join(REPO, "oracle"))
-        import oracle as O
-
+        O = oracle_module()
         o = O.Snap(8, 2, 5.0, 0.0, 0.99363, [0.5, 0.5], [1.0, 0.9], switch_flag=True)
-        t0 = time.perf_counter()
-        na = 0
+        prep = []
         for s in systems[:cpu_configs]:
             spc = np.array([names.index(x) for x in s.symbols], np.int32)
             off, nbr, R, r = o.neighborlist(s.positions, s.box, spc, mode=O.SNAP_LIST_CUTOFF)
-            o.all_descriptors(s.positions, spc, off, nbr, r)
-            na += len(s)
-        tc = time.perf_counter() - t0
-        res["cpu_baseline"] = {"atoms_per_s": na / tc, "cores": 1, "kind": "port",
-                               "sample": "oracle B + dB of the first %d configurations (%d atoms), 1 thread, "
-                                         "%.2f s" % (cpu_configs, na, tc)}
+            prep.append((s.positions, spc, off, nbr, r))
+        na = sum(len(p[1]) for p in prep)
+        tc, runs = median_time(lambda: [o.all_descriptors(*p) for p in prep])
+        res["cpu_baseline"] = {"atoms_per_s": na / tc, "cores": 1, "kind": "port", "runs": runs,
+                               "sample": "oracle compute_all_descriptors (B, dB, W) of the first %d "
+                                         "configuration(s) (%d atoms) over their prebuilt lists, 1 thread "
+                                         "(the descriptor routine is serial), median of %d runs: %.3f s"
+                                         % (cpu_configs, na, runs, tc)}
     return res
 
 
@@ -322,6 +417,7 @@ def lj_bench(steps, warmup):
     s, pot, L, ctx, lst, dsys, arr, terms, half, kern_p, keep = lj_setup()
     pos, dsp, e, f, v, _ = keep
     res = {"natoms": len(s), "half_pairs": half}
+    outs = {}
     for prec, nm in ((1, "fp32"), (0, "fp64")):
         kern = lambda: kern_p(prec)
 
@@ -337,6 +433,7 @@ def lj_bench(steps, warmup):
         ka = kernel_avg([kn])
         kms = ka[kn][1] if kn in ka else t_k * 1e3
         t_e = time_steps(e2e, max(3, steps // 2), 1)
+        outs[nm] = (float(e.download()[0]), f.download(), v.download())
         b = perfmodel.pair_bytes(len(s), half)
         res[nm] = {
             "pairs_per_s_kernel": half / (kms * 1e-3),
@@ -344,9 +441,16 @@ def lj_bench(steps, warmup):
             "kernel_ms": kms,
             "end_to_end_ms": t_e * 1e3,
             "roofline": {"bound": "hbm", "achieved": b / (kms * 1e-3) / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                         "frac": b / (kms * 1e-3) / 1e9 / PEAK_HBM_GBS, "traffic": read_traffic(kn),
-                         "algorithmic_bytes": b},
+                         "frac": b / (kms * 1e-3) / 1e9 / PEAK_HBM_GBS, "traffic": read_traffic(kn)[0],
+                         "traffic_source": read_traffic(kn)[1], "algorithmic_bytes": b},
         }
+    (e32, f32, v32), (e64, f64, v64) = outs["fp32"], outs["fp64"]
+    res["fp32_vs_fp64_error"] = {
+        "force_max": float(np.abs(f32 - f64).max()), "force_rms": float(np.sqrt(np.mean((f32 - f64) ** 2))),
+        "force_scale_max": float(np.abs(f64).max()), "force_scale_rms": float(np.sqrt(np.mean(f64 ** 2))),
+        "energy_abs": abs(e32 - e64), "energy_rel": abs(e32 - e64) / abs(e64),
+        "virial_max": float(np.abs(v32 - v64).max()), "virial_scale_max": float(np.abs(v64).max()),
+        "units": "hartree, hartree/bohr"}
     check(L.ipg_nlist_destroy(lst))
     # MD-style step over a Verlet-skin list (skin 0.5 A, positions unchanged,
     # so no rebuild): displacement check + FP64 evaluation, against the
@@ -407,8 +511,10 @@ def probe():
 
 def read_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed counter summary
-    (profiles/*_traffic.txt, lines 'kernel: NAME' and
-    'traffic_bytes_per_launch: X'), or None."""
+    (profiles/*_traffic.txt, lines 'kernel: NAME',
+    'traffic_bytes_per_launch: X' and 'measured: DATE'), and a label naming
+    that file and its date: the figure comes from a separate counter run of
+    bench.py --probe, not from this run.  (None, None) when absent."""
     import glob
 
     for fn in sorted(glob.glob(os.path.join(REPO, "profiles", "*_traffic.txt"))):
@@ -419,8 +525,10 @@ def read_traffic(kernel):
                     k, v = line.split(":", 1)
                     kv[k.strip()] = v.strip()
         if kv.get("kernel") == kernel and "traffic_bytes_per_launch" in kv:
-            return float(kv["traffic_bytes_per_launch"])
-    return None
+            src = "profiles/%s (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE of bench.py --probe, measured %s)" % (
+                os.path.basename(fn), kv.get("measured", "date not recorded"))
+            return float(kv["traffic_bytes_per_launch"]), src
+    return None, None
 
 
 def main():
@@ -549,10 +657,13 @@ def main():
         "data": "synthetic (seeded bcc supercell, seeded beta)",
         "config": {"workload": WORKLOAD, "natoms": n, "twojmax": 8, "rcut": 5.0, "species": 2,
                    "pairs": npairs, "switch_flag": True, "list_mode": "cutoff"},
-        "roofline": {"bound": "mfma", "achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": ach / PEAK_FP32_TFLOPS, "traffic": read_traffic(dom), "kernel": dom,
+        "roofline": {"bound": "valu", "achieved": ach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": ach / PEAK_FP32_TFLOPS, "traffic": read_traffic(dom)[0],
+                     "traffic_source": read_traffic(dom)[1], "kernel": dom,
                      "kernel_ms": dom_ms, "algorithmic_flops": flops[dom],
-                     "note": "VALU-bound FP32 kernel priced against the published dense FP32 peak"},
+                     "note": "FP32 vector (VALU) kernel, no MFMA, priced against the published FP32 peak "
+                             "(157.3 TFLOP/s, which counts packed FP32); the code is built without packed "
+                             "FP32, whose ceiling is 78.6 TFLOP/s"},
         "snap": {
             "fp32_adjoint_atoms_per_s": n / t32,
             "fp64_plain_atoms_per_s": n / t64,
@@ -575,16 +686,24 @@ def main():
         out["other_workloads"] = small_configs_bench(max(5, a.steps // 2))
     if not a.no_batch:
         log("batched SNAP descriptors")
-        out["batch"] = batch_bench(max(3, a.steps // 4), 1, a.batch_configs, 0 if a.no_cpu else 2)
+        out["batch"] = batch_bench(max(3, a.steps // 4), 1, a.batch_configs, 0 if a.no_cpu else 1)
     if not a.no_cpu:
         log("CPU baseline (oracle)")
-        cb = cpu_baseline_snap(s, beta, sp, a.cpu_sample)
+        hi = host_info()
+        cb = cpu_baseline_snap(s, beta, sp, a.cpu_sample, hi)
         out["cpu_baseline"] = {"value": cb["rate"], "unit": "atoms/s", "cores": cb["threads"], "kind": "port",
                                "sample": "oracle (C restatement of the reference's plain route: B, per-pair dB, "
-                                         "beta . dB) SNAP 2J=8 E+F over all %d atoms on %d OpenMP threads, %.2f s"
-                                         % (n, cb["threads"], cb["seconds"]),
+                                         "beta . dB) SNAP 2J=8 E+F over all %d atoms on %d OpenMP threads, median "
+                                         "of %d runs after a warm-up: %.3f s" % (n, cb["threads"], cb["runs"],
+                                                                                  cb["seconds"]),
                                "value_1_thread": cb["rate1"],
-                               "sample_1_thread": "first %d atoms, 1 thread, %.2f s" % (cb["sample1"], cb["seconds1"])}
+                               "sample_1_thread": "first %d atoms, 1 thread, median of %d runs: %.3f s"
+                                                  % (cb["sample1"], cb["runs"], cb["seconds1"]),
+                               "host": hi}
+        if not a.no_lj:
+            out["cpu_baseline_pairs"] = cpu_baseline_pairs(hi)
+            out["cpu_baseline_pairs"]["host"] = hi
+            out["cpu_baseline_pairs"]["kind"] = "port"
     print(json.dumps(out))
 
 

@@ -6,6 +6,9 @@
  * reference file:line it follows.  Build with -ffp-contract=off.
  */
 #include "ipg_oracle.h"
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #include <math.h>
 #include <stdlib.h>
@@ -110,10 +113,62 @@ int64_t orc_nlist_brute(const double* pos, int64_t n, const double L[3], double 
  * (fewer when a dimension has < 3 cells), and membership uses the same
  * PeriodicEuclidean formula on the original coordinates.  Output equals
  * orc_nlist_brute (sorted j). */
-int64_t orc_nlist_cells(const double* pos, int64_t n, const double L[3], double rcut, int kind,
-                        int64_t* offsets, int32_t* nbr, double* disp, int64_t cap) {
+/* Hits of atom i in the cell grid (the candidates of the 27 surrounding
+ * cells, or every cell of a periodic dimension with fewer than 3), sorted;
+ * tmp has room for n entries.  Returns the count. */
+typedef struct {
+  const double* pos;
+  const double* L;
+  double rcut;
+  int kind, nc[3];
+  const int32_t* cell;
+  const int64_t* cstart;
+  const int32_t* cat;
+} orc_grid;
+
+static int64_t cells_row(const orc_grid* g, int64_t i, int32_t* tmp) {
+  const int* nc = g->nc;
+  const double* L = g->L;
+  int c0 = g->cell[i] % nc[0], c1 = (g->cell[i] / nc[0]) % nc[1], c2 = g->cell[i] / (nc[0] * nc[1]);
+  int lo_[3], hi_[3], cc[3] = {c0, c1, c2};
+  for (int d = 0; d < 3; d++) {
+    if (nc[d] >= 3 || isinf(L[d])) { lo_[d] = cc[d] - 1; hi_[d] = cc[d] + 1; }
+    else { lo_[d] = 0; hi_[d] = nc[d] - 1; }
+  }
+  int64_t cnt = 0;
+  for (int z = lo_[2]; z <= hi_[2]; z++) {
+    int zz = z;
+    if (isinf(L[2])) { if (zz < 0 || zz >= nc[2]) continue; }
+    else if (nc[2] >= 3) zz = (zz + nc[2]) % nc[2];
+    for (int y = lo_[1]; y <= hi_[1]; y++) {
+      int yy = y;
+      if (isinf(L[1])) { if (yy < 0 || yy >= nc[1]) continue; }
+      else if (nc[1] >= 3) yy = (yy + nc[1]) % nc[1];
+      for (int x = lo_[0]; x <= hi_[0]; x++) {
+        int xx = x;
+        if (isinf(L[0])) { if (xx < 0 || xx >= nc[0]) continue; }
+        else if (nc[0] >= 3) xx = (xx + nc[0]) % nc[0];
+        int64_t cid = ((int64_t)zz * nc[1] + yy) * nc[0] + xx;
+        for (int64_t s = g->cstart[cid]; s < g->cstart[cid + 1]; s++) {
+          int32_t j = g->cat[s];
+          if (j == i) continue;
+          if (g->kind == ORC_LIST_HALF && j < i) continue;
+          if (pe_within(g->pos + 3 * i, g->pos + 3 * j, L, g->rcut)) tmp[cnt++] = j;
+        }
+      }
+    }
+  }
+  qsort(tmp, (size_t)cnt, sizeof(int32_t), cmp_i32);
+  return cnt;
+}
+
+/* Cell-binned list; rows are independent, so nthreads > 1 (OpenMP) counts
+ * and fills them in parallel with the same result. */
+static int64_t nlist_cells_mt(const double* pos, int64_t n, const double L[3], double rcut, int kind,
+                              int64_t* offsets, int32_t* nbr, double* disp, int64_t cap, int nthreads) {
   if (n < 0 || !(rcut > 0)) return -1;
   if (n == 0) { offsets[0] = 0; return 0; }
+  if (nthreads < 1) nthreads = 1;
   double lo[3], w[3];
   int nc[3];
   for (int d = 0; d < 3; d++) {
@@ -159,51 +214,50 @@ int64_t orc_nlist_cells(const double* pos, int64_t n, const double L[3], double 
   for (int64_t i = 0; i < n; i++) cat[fillp[cell[i]]++] = (int32_t)i;
   free(fillp);
 
-  int32_t* tmp = (int32_t*)malloc(sizeof(int32_t) * n);
+  orc_grid g = {pos, L, rcut, kind, {nc[0], nc[1], nc[2]}, cell, cstart, cat};
+  int32_t* tmp = (int32_t*)malloc(sizeof(int32_t) * n * (size_t)nthreads);
+  /* pass 1: row lengths */
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 256)
+  for (int64_t i = 0; i < n; i++) {
+    int t = 0;
+#ifdef _OPENMP
+    t = omp_get_thread_num();
+#endif
+    offsets[i] = cells_row(&g, i, tmp + (size_t)t * n);
+  }
   int64_t np = 0;
   for (int64_t i = 0; i < n; i++) {
+    int64_t c = offsets[i];
     offsets[i] = np;
-    int c0 = cell[i] % nc[0], c1 = (cell[i] / nc[0]) % nc[1], c2 = cell[i] / (nc[0] * nc[1]);
-    int lo_[3], hi_[3], cc[3] = {c0, c1, c2};
-    for (int d = 0; d < 3; d++) {
-      if (nc[d] >= 3 || isinf(L[d])) { lo_[d] = cc[d] - 1; hi_[d] = cc[d] + 1; }
-      else { lo_[d] = 0; hi_[d] = nc[d] - 1; }
-    }
-    int64_t cnt = 0;
-    for (int z = lo_[2]; z <= hi_[2]; z++) {
-      int zz = z;
-      if (isinf(L[2])) { if (zz < 0 || zz >= nc[2]) continue; }
-      else if (nc[2] >= 3) zz = (zz + nc[2]) % nc[2];
-      for (int y = lo_[1]; y <= hi_[1]; y++) {
-        int yy = y;
-        if (isinf(L[1])) { if (yy < 0 || yy >= nc[1]) continue; }
-        else if (nc[1] >= 3) yy = (yy + nc[1]) % nc[1];
-        for (int x = lo_[0]; x <= hi_[0]; x++) {
-          int xx = x;
-          if (isinf(L[0])) { if (xx < 0 || xx >= nc[0]) continue; }
-          else if (nc[0] >= 3) xx = (xx + nc[0]) % nc[0];
-          int64_t cid = ((int64_t)zz * nc[1] + yy) * nc[0] + xx;
-          for (int64_t s = cstart[cid]; s < cstart[cid + 1]; s++) {
-            int32_t j = cat[s];
-            if (j == i) continue;
-            if (kind == ORC_LIST_HALF && j < i) continue;
-            if (pe_within(pos + 3 * i, pos + 3 * j, L, rcut)) tmp[cnt++] = j;
-          }
+    np += c;
+  }
+  offsets[n] = np;
+  /* pass 2: rows (entries past cap are dropped) */
+  if (nbr || disp) {
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 256)
+    for (int64_t i = 0; i < n; i++) {
+      int t = 0;
+#ifdef _OPENMP
+      t = omp_get_thread_num();
+#endif
+      int32_t* row = tmp + (size_t)t * n;
+      int64_t cnt = cells_row(&g, i, row);
+      for (int64_t k = 0; k < cnt; k++) {
+        int64_t q = offsets[i] + k;
+        if (q < cap) {
+          if (nbr) nbr[q] = row[k];
+          if (disp) orc_get_displacement(L, pos + 3 * i, pos + 3 * row[k], disp + 3 * q);
         }
       }
     }
-    qsort(tmp, (size_t)cnt, sizeof(int32_t), cmp_i32);
-    for (int64_t k = 0; k < cnt; k++) {
-      if (np + k < cap) {
-        if (nbr) nbr[np + k] = tmp[k];
-        if (disp) orc_get_displacement(L, pos + 3 * i, pos + 3 * tmp[k], disp + 3 * (np + k));
-      }
-    }
-    np += cnt;
   }
-  offsets[n] = np;
   free(tmp); free(cat); free(cstart); free(cell);
   return np;
+}
+
+int64_t orc_nlist_cells(const double* pos, int64_t n, const double L[3], double rcut, int kind,
+                        int64_t* offsets, int32_t* nbr, double* disp, int64_t cap) {
+  return nlist_cells_mt(pos, n, L, rcut, kind, offsets, nbr, disp, cap, 1);
 }
 
 /* ================================================================== */
@@ -295,29 +349,29 @@ static double norm3(const double r[3]) { return sqrt(r[0] * r[0] + r[1] * r[1] +
 /* src/types/empirical_potential.jl:20-67 (_perform_pairwise with the
  * energy_and_force and virial_stress closures) for each term, summed with
  * the term coefficient as linear_combination_potential.jl:27-39 does. */
-int orc_pair_energy_force_virial(const double* pos, int64_t n, const double L[3], const int32_t* species,
-                                 const orc_pair_term* terms, int nterms, int use_cells,
-                                 double* e, double* f, double* v) {
-  if (n < 0 || nterms < 0) return -1;
-  if (e) *e = 0.0;
-  if (f) memset(f, 0, sizeof(double) * 3 * n);
-  if (v) memset(v, 0, sizeof(double) * 6);
-  int64_t* off = (int64_t*)malloc(sizeof(int64_t) * (n + 1));
-  double* ft = (double*)malloc(sizeof(double) * 3 * (n > 0 ? n : 1));
-  for (int k = 0; k < nterms; k++) {
-    const orc_pair_term* t = &terms[k];
-    int64_t np = use_cells ? orc_nlist_cells(pos, n, L, t->rcut, ORC_LIST_HALF, off, NULL, NULL, 0)
-                           : orc_nlist_brute(pos, n, L, t->rcut, ORC_LIST_HALF, off, NULL, NULL, 0);
-    if (np < 0) { free(off); free(ft); return -1; }
-    int32_t* nbr = (int32_t*)malloc(sizeof(int32_t) * (np > 0 ? np : 1));
-    double* disp = (double*)malloc(sizeof(double) * 3 * (np > 0 ? np : 1));
-    if (use_cells) orc_nlist_cells(pos, n, L, t->rcut, ORC_LIST_HALF, off, nbr, disp, np);
-    else orc_nlist_brute(pos, n, L, t->rcut, ORC_LIST_HALF, off, nbr, disp, np);
+/* empirical_potential.jl:20-67 for one term over its own half list: the
+ * reference splits the atoms over threads with per-thread energy, force
+ * and virial accumulators (:28-46); here thread t takes a contiguous block
+ * of atoms and the per-thread sums are added in thread order, so a given
+ * thread count always gives the same rounding. */
+static void pair_term_mt(const orc_pair_term* t, int64_t n, const int32_t* species, const int64_t* off,
+                         const int32_t* nbr, const double* disp, int nthreads, double* et_out, double* ft_out,
+                         double vt_out[6]) {
+  double* ftt = (double*)calloc((size_t)3 * n * nthreads, sizeof(double));
+  double* etv = (double*)calloc((size_t)7 * nthreads, sizeof(double));
+#pragma omp parallel num_threads(nthreads)
+  {
+    int tid = 0, nt = 1;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+    nt = omp_get_num_threads();
+#endif
+    int64_t a0 = n * tid / nt, a1 = n * (tid + 1) / nt;
+    double* ft = ftt + (size_t)3 * n * tid;
     double et = 0.0, vt[6] = {0, 0, 0, 0, 0, 0};
-    memset(ft, 0, sizeof(double) * 3 * n);
-    for (int64_t i = 0; i < n; i++) {
-      /* empirical_potential.jl:36-38: a pair counts iff both species are in
-       * get_species(p) (the fast path of :26 gives the same set). */
+    for (int64_t i = a0; i < a1; i++) {
+      /* :36-38: a pair counts iff both species are in get_species(p) (the
+       * fast path of :26 gives the same set). */
       if (!((t->species_mask >> species[i]) & 1ull)) continue;
       for (int64_t q = off[i]; q < off[i + 1]; q++) {
         int32_t j = nbr[q];
@@ -326,7 +380,7 @@ int orc_pair_energy_force_virial(const double* pos, int64_t n, const double L[3]
         double R = norm3(r);             /* :17-18 potential_energy/force(r) */
         double fo[3];
         orc_pair_F(t, R, r, fo);
-        if (e) et = et + orc_pair_V(t, R); /* :51-56 */
+        et = et + orc_pair_V(t, R);      /* :51-56 */
         for (int d = 0; d < 3; d++) {
           ft[3 * i + d] += fo[d];
           ft[3 * j + d] -= fo[d];
@@ -340,6 +394,44 @@ int orc_pair_energy_force_virial(const double* pos, int64_t n, const double L[3]
         vt[5] += r[1] * fo[0];
       }
     }
+    etv[7 * tid] = et;
+    for (int k = 0; k < 6; k++) etv[7 * tid + 1 + k] = vt[k];
+  }
+  *et_out = 0.0;
+  for (int k = 0; k < 6; k++) vt_out[k] = 0.0;
+  memset(ft_out, 0, sizeof(double) * 3 * n);
+  for (int tid = 0; tid < nthreads; tid++) {
+    *et_out += etv[7 * tid];
+    for (int k = 0; k < 6; k++) vt_out[k] += etv[7 * tid + 1 + k];
+    const double* ft = ftt + (size_t)3 * n * tid;
+    for (int64_t a = 0; a < 3 * n; a++) ft_out[a] += ft[a];
+  }
+  free(ftt);
+  free(etv);
+}
+
+int orc_pair_energy_force_virial_mt(const double* pos, int64_t n, const double L[3], const int32_t* species,
+                                    const orc_pair_term* terms, int nterms, int use_cells, int nthreads,
+                                    double* e, double* f, double* v) {
+  if (n < 0 || nterms < 0) return -1;
+  if (nthreads < 1) nthreads = 1;
+  if (e) *e = 0.0;
+  if (f) memset(f, 0, sizeof(double) * 3 * n);
+  if (v) memset(v, 0, sizeof(double) * 6);
+  int64_t* off = (int64_t*)malloc(sizeof(int64_t) * (n + 1));
+  double* ft = (double*)malloc(sizeof(double) * 3 * (n > 0 ? n : 1));
+  for (int k = 0; k < nterms; k++) {
+    /* linear_combination_potential.jl:27-39: every term rebuilds its list */
+    const orc_pair_term* t = &terms[k];
+    int64_t np = use_cells ? nlist_cells_mt(pos, n, L, t->rcut, ORC_LIST_HALF, off, NULL, NULL, 0, nthreads)
+                           : orc_nlist_brute(pos, n, L, t->rcut, ORC_LIST_HALF, off, NULL, NULL, 0);
+    if (np < 0) { free(off); free(ft); return -1; }
+    int32_t* nbr = (int32_t*)malloc(sizeof(int32_t) * (np > 0 ? np : 1));
+    double* disp = (double*)malloc(sizeof(double) * 3 * (np > 0 ? np : 1));
+    if (use_cells) nlist_cells_mt(pos, n, L, t->rcut, ORC_LIST_HALF, off, nbr, disp, np, nthreads);
+    else orc_nlist_brute(pos, n, L, t->rcut, ORC_LIST_HALF, off, nbr, disp, np);
+    double et = 0.0, vt[6];
+    pair_term_mt(t, n, species, off, nbr, disp, nthreads, &et, ft, vt);
     if (e) *e += t->coeff * et;
     if (f) for (int64_t a = 0; a < 3 * n; a++) f[a] += t->coeff * ft[a];
     if (v) for (int a = 0; a < 6; a++) v[a] += t->coeff * vt[a];
@@ -347,6 +439,12 @@ int orc_pair_energy_force_virial(const double* pos, int64_t n, const double L[3]
   }
   free(off); free(ft);
   return 0;
+}
+
+int orc_pair_energy_force_virial(const double* pos, int64_t n, const double L[3], const int32_t* species,
+                                 const orc_pair_term* terms, int nterms, int use_cells,
+                                 double* e, double* f, double* v) {
+  return orc_pair_energy_force_virial_mt(pos, n, L, species, terms, nterms, use_cells, 1, e, f, v);
 }
 
 /* ================================================================== */

@@ -18,18 +18,26 @@
  *   orc_pair_*                    -- test/unit/types/empirical_potential.jl
  *                                    (mock potential, species filter, virial),
  *                                    LJ 150/1000 cluster known answers, closed
- *                                    forms, finite differences of V.
+ *                                    forms, finite differences of V; Born-Mayer
+ *                                    by FD of its exponential part and an exact
+ *                                    value of its power part; ZBL with e != 1
+ *                                    against the textbook screening function
+ *                                    at d ln e and its derivative.
  *   orc_snap_tables_*             -- sympy exact Clebsch-Gordan, closed-form
  *                                    table sizes and rootpq.
  *   orc_snap_atom_*, descriptors  -- Wigner-D unitarity and closed form,
  *                                    rotation/permutation invariance of B,
  *                                    numpy brute-force bispectrum, central
- *                                    finite differences for dB.
+ *                                    finite differences for dB; bzero as the
+ *                                    isolated atom's B (B - bzero = 0); the
+ *                                    as-is chem_flag path on isolated atoms
+ *                                    against its hand-derived closed form.
  *   orc_snap_adjoint_*            -- equality with beta . dB (plain path) and
  *                                    finite differences of E.
- *   orc_snap_neighborlist         -- brute-force restatement only: parity
- *                                    unpinned beyond the shared distance
- *                                    formula (no reference golden list).
+ *   orc_snap_neighborlist         -- a hand-derived list of a 6-atom,
+ *                                    two-species, mixed-boundary system in
+ *                                    both modes (tests/golden/
+ *                                    snap_list_hand.json).
  */
 #ifndef IPG_ORACLE_H
 #define IPG_ORACLE_H
@@ -91,6 +99,14 @@ void orc_pair_F(const orc_pair_term* t, double R, const double r[3], double out[
 int orc_pair_energy_force_virial(const double* pos, int64_t n, const double L[3], const int32_t* species,
                                  const orc_pair_term* terms, int nterms, int use_cells,
                                  double* e, double* f, double* v);
+/* The same over nthreads OpenMP threads, as the reference threads
+ * _perform_pairwise (empirical_potential.jl:28-46): contiguous blocks of
+ * atoms, per-thread sums added in thread order (the rounding depends on
+ * the thread count only).  The cell list rows are built in parallel too
+ * (the same list).  Used to time the CPU baseline. */
+int orc_pair_energy_force_virial_mt(const double* pos, int64_t n, const double L[3], const int32_t* species,
+                                    const orc_pair_term* terms, int nterms, int use_cells, int nthreads,
+                                    double* e, double* f, double* v);
 
 /* ------------------------------------------------------------------ */
 /* SNAP (src/BasisSystems/SNAP)                                        */

@@ -96,6 +96,8 @@ def lib():
         L.orc_pair_F.argtypes = [C.POINTER(PairTerm), d, P, P]
         L.orc_pair_energy_force_virial.restype = C.c_int
         L.orc_pair_energy_force_virial.argtypes = [P, i64, P, P, P, C.c_int, C.c_int, P, P, P]
+        L.orc_pair_energy_force_virial_mt.restype = C.c_int
+        L.orc_pair_energy_force_virial_mt.argtypes = [P, i64, P, P, P, C.c_int, C.c_int, C.c_int, P, P, P]
         L.orc_snap_tables_build.restype = C.POINTER(SnapTables)
         L.orc_snap_tables_build.argtypes = [C.c_int] * 5
         L.orc_snap_tables_free.argtypes = [C.POINTER(SnapTables)]
@@ -181,7 +183,7 @@ def make_terms(terms):
     return arr
 
 
-def pair_energy_force_virial(pos, L, species, terms, cells=False):
+def pair_energy_force_virial(pos, L, species, terms, cells=False, nthreads=1):
     pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
     n = pos.shape[0]
     sp = np.ascontiguousarray(species, dtype=np.int32)
@@ -189,8 +191,8 @@ def pair_energy_force_virial(pos, L, species, terms, cells=False):
     e = np.zeros(1)
     f = np.zeros((n, 3))
     v = np.zeros(6)
-    rc = lib().orc_pair_energy_force_virial(_p(pos), n, _p(_box(L)), _p(sp), C.cast(arr, C.c_void_p),
-                                            len(terms), int(cells), _p(e), _p(f), _p(v))
+    rc = lib().orc_pair_energy_force_virial_mt(_p(pos), n, _p(_box(L)), _p(sp), C.cast(arr, C.c_void_p),
+                                               len(terms), int(cells), int(nthreads), _p(e), _p(f), _p(v))
     if rc != 0:
         raise ValueError("invalid pair input")
     return e[0], f, v

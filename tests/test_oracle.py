@@ -537,3 +537,23 @@ def test_zbl_charge_base(orc, e):
         dPhi = (zbl_universal_screening(x + h) - zbl_universal_screening(x - h)) / (2 * h)
         F = orc.pair_F(t, R, [R, 0.0, 0.0])[0]  # r = R e_x, so F_x = F.r/R
         assert F == pytest.approx(cf * (zbl_universal_screening(x) + d * dPhi) / R ** 2, rel=1e-7)
+
+
+def test_threaded_pair_path_equals_serial(orc):
+    """The OpenMP pair path (the CPU baseline's) gives the serial path's
+    list and, up to the per-thread summation order, its results."""
+    rng = np.random.default_rng(8)
+    X = rng.random((900, 3)) * 14.0
+    sp = rng.integers(0, 2, 900).astype(np.int32)
+    L = [14.0, 14.0, INF]
+    terms = [dict(kind=orc.PAIR_LJ, rcut=3.0, p=[0.1, 1.0], species_mask=1),
+             dict(kind=orc.PAIR_MORSE, rcut=2.5, p=[0.2, 1.3, 1.1], coeff=0.5)]
+    e1, f1, v1 = orc.pair_energy_force_virial(X, L, sp, terms, cells=True)
+    e0, f0, v0 = orc.pair_energy_force_virial(X, L, sp, terms, cells=False)
+    for nt in (1, 3, 8):
+        e, f, v = orc.pair_energy_force_virial(X, L, sp, terms, cells=True, nthreads=nt)
+        if nt == 1:
+            assert e == e1 and np.array_equal(f, f1) and np.array_equal(v, v1)
+        assert e == pytest.approx(e0, rel=1e-12)
+        np.testing.assert_allclose(f, f0, rtol=0, atol=1e-12 * np.abs(f0).max())
+        np.testing.assert_allclose(v, v0, rtol=0, atol=1e-12 * np.abs(v0).max())
