@@ -178,6 +178,24 @@ int ipg_pair_energy_forces_dev(ipg_ctx* ctx, const ipg_system* sys, const ipg_pa
 int ipg_pair_eval_dev(ipg_ctx* ctx, const ipg_nlist* full_list, const ipg_system* sys, const ipg_pair_term* terms,
                       int nterms, int precision, double* energy, double* forces, double* virial);
 
+/* The pair kernel over a prebuilt device full list, as ipg_pair_eval_dev,
+ * without any device allocation: for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Its partial sums and
+ * status word live in a static device array of 1 MiB per GPU, held until
+ * the process ends; a list whose pair kernel needs more than
+ * (1 MiB - 512) / 56 = 18715 blocks (about 299k atoms at 48 or more
+ * neighbors each, 1.19M atoms below 12) returns IPG_ERR_INVALID.  The
+ * results equal ipg_pair_eval_dev's bit for bit.  It does not synchronize
+ * and reports no device status: species indices outside [0, 64) make an
+ * atom take part in no term (no kernel reads outside its tables).  Every
+ * call on a GPU shares that GPU's array, so calls that overlap on one GPU
+ * (other threads, other streams) race; calls on one stream are ordered.
+ * Profiling events are not recorded. */
+int ipg_pair_eval_noalloc_dev(ipg_ctx* ctx, const ipg_nlist* full_list, const ipg_system* sys,
+                              const ipg_pair_term* terms, int nterms, int precision, double* energy, double* forces,
+                              double* virial);
+
 /* ------------------------------------------------------------------ */
 /* SNAP                                                                */
 /* ------------------------------------------------------------------ */

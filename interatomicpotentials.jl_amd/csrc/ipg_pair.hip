@@ -14,6 +14,8 @@
 // adds +F and hi adds -F, exactly the reference's f[i] += fo, f[j] -= fo.
 // Energy and virial are taken from the lo side only (half-pair sums) and
 // reduced in a fixed tree.
+#include <mutex>
+
 #include "ipg_nlist.h"
 
 namespace ipg {
@@ -276,14 +278,20 @@ static int pick_lanes(int64_t n, int64_t npairs) {
   return 4;
 }
 
-// Launch the pair kernel over a full list; e/v to d_ev[7] (device).
+// Blocks of the pair kernel over a list of npairs (7 partial sums each).
+static int64_t pair_blocks(int64_t n, int64_t npairs) {
+  const int64_t nblk = (n * pick_lanes(n, npairs) + 255) / 256;
+  return nblk < 1 ? 1 : nblk;
+}
+
+// Launch the pair kernel over a full list; e/v to d_ev[7] (device).  part
+// (pair_blocks() rows of 7 doubles) holds the per-block partial sums; null
+// takes it from the context's scratch.
 static void run_pair(ipg_ctx* ctx, const DevSystem& s, const DevList& L, const PairTable& tab, int precision,
-                     double* d_forces, double* d_ev, int32_t* flag) {
+                     double* d_forces, double* d_ev, int32_t* flag, double* part = nullptr) {
   int lanes = pick_lanes(s.n, L.npairs);
-  int64_t nthreads = s.n * lanes;
-  int64_t nblk = (nthreads + 255) / 256;
-  if (nblk < 1) nblk = 1;
-  double* part = scratch_t<double>(ctx, "pair_part", nblk * 7);
+  const int64_t nblk = pair_blocks(s.n, L.npairs);
+  if (!part) part = scratch_t<double>(ctx, "pair_part", nblk * 7);
   auto go = [&](auto tag, auto lanes_c) {
     using T = decltype(tag);
     constexpr int LN = decltype(lanes_c)::value;
@@ -355,6 +363,59 @@ static void check_pair_flag(int32_t over) {
   if (over & 16) fail(IPG_ERR_INVALID, "species index out of range [0, 64)");
 }
 
+// ---------------------------------------------------------------------------
+// allocation-free evaluation over a prebuilt list
+// ---------------------------------------------------------------------------
+// The device memory of ipg_pair_eval_noalloc_dev beyond its inputs and
+// outputs: the status word (bytes 0..3), the energy/virial total (7
+// doubles at byte 256) and the per-block partial sums (7 doubles per block
+// of the pair kernel, from byte 512).  Every GPU holds its own copy for the
+// life of the process; all calls on one GPU share it.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)1 << 20;
+constexpr size_t NOALLOC_PART_OFFSET = 512;
+
+}  // namespace ipg
+
+// at global scope: inside namespace ipg the name scratch is the context's
+// scratch-buffer function
+static __device__ __align__(256) unsigned char scratch[ipg::NOALLOC_SCRATCH_BYTES];
+
+namespace ipg {
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_noalloc_mu;
+static void* g_noalloc_ptr[NOALLOC_MAX_DEVICES];
+
+// Device address of this GPU's copy of the static scratch, looked up once
+// per device on its first call (the device must be current).  Host code
+// uses this address, never the array's name.
+static char* noalloc_scratch(int device) {
+  IPG_REQUIRE(device >= 0 && device < NOALLOC_MAX_DEVICES, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (!g_noalloc_ptr[device]) {
+    void* p = nullptr;
+    IPG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(::scratch)));
+    IPG_REQUIRE(p != nullptr, "hipGetSymbolAddress returned no address for the static scratch");
+    g_noalloc_ptr[device] = p;
+  }
+  return static_cast<char*>(g_noalloc_ptr[device]);
+}
+
+// Largest pair-kernel block count the static scratch holds.
+constexpr int64_t NOALLOC_MAX_BLOCKS = (int64_t)((NOALLOC_SCRATCH_BYTES - NOALLOC_PART_OFFSET) / (7 * sizeof(double)));
+// reduce_rows sums up to 65536 rows of width 7 in one block, with no
+// buffer of its own
+static_assert(NOALLOC_MAX_BLOCKS <= 65536, "partial-sum rows beyond the single-block reduction");
+
+// Profiling events are created on demand, so an allocation-free call
+// records none; the context's setting is restored on every exit.
+struct ProfOff {
+  ipg_ctx* ctx;
+  bool was;
+  explicit ProfOff(ipg_ctx* c) : ctx(c), was(c->prof) { c->prof = false; }
+  ~ProfOff() { ctx->prof = was; }
+};
+
 // Device-buffer form: the flag is read here (one synchronization).
 static void pair_dev_checked(ipg_ctx* ctx, const DevSystem& s, const ipg_pair_term* terms, int nterms, int precision,
                              double* d_energy, double* d_forces, double* d_virial) {
@@ -410,6 +471,40 @@ int ipg_pair_energy_forces_dev(ipg_ctx* ctx, const ipg_system* sys, const ipg_pa
     IPG_HIP(hipSetDevice(ctx->device));
     DevSystem s = make_dev_system(sys);
     pair_dev_checked(ctx, s, terms, nterms, precision, energy, forces, virial);
+  });
+}
+
+int ipg_pair_eval_noalloc_dev(ipg_ctx* ctx, const ipg_nlist* full_list, const ipg_system* sys,
+                              const ipg_pair_term* terms, int nterms, int precision, double* energy, double* forces,
+                              double* virial) {
+  return guard([&] {
+    IPG_REQUIRE(ctx && sys && full_list, "NULL argument");
+    IPG_REQUIRE(full_list->kind == IPG_LIST_FULL, "ipg_pair_eval_noalloc_dev needs an IPG_LIST_FULL list");
+    IPG_REQUIRE(precision == IPG_FP64 || precision == IPG_FP32, "precision must be IPG_FP64 or IPG_FP32");
+    IPG_REQUIRE(max_rcut(terms, nterms) <= full_list->cut, "list cutoff is below a term's rcut");
+    IPG_HIP(hipSetDevice(ctx->device));
+    DevSystem s = make_dev_system(sys);
+    const DevList& L = full_list->list;
+    IPG_REQUIRE(L.n == s.n, "list natoms does not match the system");
+    const int64_t nblk = pair_blocks(s.n, L.npairs);
+    if (nblk > NOALLOC_MAX_BLOCKS)
+      fail(IPG_ERR_INVALID, "system too large for the allocation-free pair evaluation: " + std::to_string(nblk) +
+                                " pair-kernel blocks need " +
+                                std::to_string(NOALLOC_PART_OFFSET + (size_t)nblk * 7 * sizeof(double)) +
+                                " bytes of its static scratch of " + std::to_string(NOALLOC_SCRATCH_BYTES));
+    PairTable tab = prepare_terms(terms, nterms, full_list->rcut);
+    char* base = noalloc_scratch(ctx->device);
+    int32_t* flag = reinterpret_cast<int32_t*>(base);
+    double* ev = reinterpret_cast<double*>(base + 256);
+    double* part = reinterpret_cast<double*>(base + NOALLOC_PART_OFFSET);
+    ProfOff quiet(ctx);
+    if (s.n == 0) {
+      zero(ctx, ev, sizeof(double) * 8);
+    } else {
+      zero(ctx, flag, sizeof(int32_t));  // a fill kernel: the previous call's status stays in the array
+      run_pair(ctx, s, L, tab, precision, forces, ev, flag, part);
+    }
+    launch(ctx, "split_ev", [&] { k_split_ev<<<1, 64, 0, ctx->stream>>>(ev, energy, virial); });
   });
 }
 

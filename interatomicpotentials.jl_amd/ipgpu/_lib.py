@@ -34,7 +34,7 @@ EXPORTED = [
     "ipg_snap_energy_forces_dev", "ipg_snap_force_vjp", "ipg_ctx_set_graphs", "ipg_snap_graph_info",
     "ipg_device_copy", "ipg_snap_batch_descriptors", "ipg_snap_batch_descriptors_dev", "ipg_snap_batch_info",
     "ipg_nlist_build_skin_dev", "ipg_nlist_update_dev", "ipg_prof_select", "ipg_snap_virial_stress",
-    "ipg_snap_energy_forces_virial",
+    "ipg_snap_energy_forces_virial", "ipg_pair_eval_noalloc_dev",
 ]
 
 
@@ -118,6 +118,7 @@ def load():
             "ipg_pair_energy_forces": (C.c_int, [P, pS, P, C.c_int, C.c_int, P, P, P]),
             "ipg_pair_energy_forces_dev": (C.c_int, [P, pS, P, C.c_int, C.c_int, P, P, P]),
             "ipg_pair_eval_dev": (C.c_int, [P, P, pS, P, C.c_int, C.c_int, P, P, P]),
+            "ipg_pair_eval_noalloc_dev": (C.c_int, [P, P, pS, P, C.c_int, C.c_int, P, P, P]),
             "ipg_snap_create": (C.c_int, [P, C.POINTER(SnapParams), C.POINTER(P)]),
             "ipg_snap_destroy": (C.c_int, [P]),
             "ipg_snap_info": (C.c_int, [P, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]),
