@@ -47,7 +47,7 @@ extern "C" {
 #define IPG_ERR_INVALID (-1)     /* invalid argument (n < 0, rcut <= 0, NaN, bad box, ...) */
 #define IPG_ERR_CAPACITY (-2)    /* output capacity too small; *npairs holds the size needed */
 #define IPG_ERR_HIP (-3)         /* HIP runtime error */
-#define IPG_ERR_UNSUPPORTED (-4) /* option not implemented on the GPU (e.g. chem_flag) */
+#define IPG_ERR_UNSUPPORTED (-4) /* option not available (e.g. twojmax > 10, an adjoint path with chem_flag) */
 #define IPG_ERR_NOMEM (-5)       /* device allocation failed */
 
 typedef struct ipg_ctx ipg_ctx;
@@ -218,7 +218,16 @@ enum { IPG_SNAP_PLAIN_FP64 = 0, IPG_SNAP_ADJOINT_FP64 = 1, IPG_SNAP_ADJOINT_FP32
 /* Replaces SNAP(...) construction + initialize_prebuilt_arrays
  * (src/BasisSystems/SNAP/snap.jl:20-36, types/prebuilt_arrays.jl:48-321):
  * builds the index, Clebsch-Gordan and rootpq tables in FP64 and uploads
- * them.  twojmax must be in [0, 8]; chem_flag must be 0 on the GPU. */
+ * them.  twojmax in [0, 10] (from 11 on the reference's Int64 factorial
+ * products in init_clebsch_gordan overflow and it stops with an error:
+ * IPG_ERR_UNSUPPORTED).  chem_flag = true (bnorm_flag must equal it,
+ * snap.jl:24-27) and twojmax 9, 10 run runtime-table FP64 kernels that
+ * reproduce the reference's multi-element arithmetic as written
+ * (compute_zi accumulates every element pair into the first Z block,
+ * compute_z.jl:46-47); for those configurations the energy/force/virial
+ * calls accept IPG_SNAP_PLAIN_FP64 only (the reference's beta . dB route;
+ * other paths: IPG_ERR_UNSUPPORTED), the batched descriptors IPG_FP64
+ * only, and ipg_snap_force_vjp returns IPG_ERR_UNSUPPORTED. */
 int ipg_snap_create(ipg_ctx* ctx, const ipg_snap_params* params, ipg_snap** out);
 int ipg_snap_destroy(ipg_snap* snap);
 /* ncoeff = get_num_snap_coeffs (utl.jl:19-37); table sizes. */

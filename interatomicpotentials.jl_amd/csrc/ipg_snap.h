@@ -8,7 +8,9 @@
 
 namespace ipg {
 
-constexpr int SNAP_JMAX = 8;  // largest twojmax with a compiled kernel set
+constexpr int SNAP_JMAX = 8;       // largest twojmax with a compiled kernel set
+constexpr int SNAP_JMAX_ALL = 10;  // largest twojmax of the runtime-table kernels (ipg_snap_gen.hip): from 2J = 11
+                                   // on the reference's Int64 factorial products in init_clebsch_gordan overflow
 
 // One Z entry (types/prebuilt_arrays.jl:6-17 SNA_ZINDICES) plus the
 // Clebsch-Gordan block offset and the Z-block id.
@@ -52,6 +54,11 @@ struct SnapHost {
   std::vector<int> yjjz;                       // jjz list per left-half position
   std::vector<int> yjblk;                      // Z block of each yjjz entry
   std::vector<uint32_t> zplan;                 // compact Z loop plan, 3 words per entry
+  // runtime-table path (chem_flag = true, or twojmax > SNAP_JMAX)
+  bool generic = false, chem = false;
+  int nel = 1;                                 // prebuilt num_elements: nspecies with chem_flag, else 1
+  std::vector<double> rootpq;                  // (J+1)^2, [p][q] = sqrt(p/q) (prebuilt_arrays.jl:314-321)
+  std::vector<int> jmm;                        // idxu_max: j | mb << 8 | ma << 16 of each U entry
 };
 
 void snap_build_host(SnapHost& h, const ipg_snap_params* p);
@@ -62,9 +69,9 @@ struct SnapDev {
   int bnorm, bzero, switch_flag;
   double rcutfac, rmin0, rfac0, wself;
   double radii[8], weight[8];
-  double bzero_j[SNAP_JMAX + 1];
-  int idxu_block[SNAP_JMAX + 2];
-  int hoff[SNAP_JMAX + 2];  // offsets of each level's left-half rows (mb <= j/2)
+  double bzero_j[SNAP_JMAX_ALL + 1];
+  int idxu_block[SNAP_JMAX_ALL + 2];
+  int hoff[SNAP_JMAX_ALL + 2];  // offsets of each level's left-half rows (mb <= j/2)
   const ZEnt* zent;
   const YBlk* yblk;
   const BTerm* bterm;
@@ -76,6 +83,12 @@ struct SnapDev {
   const int* yjblk;        // Z block of each yjjz entry
   const uint32_t* zplan;   // 3 words per Z entry, see snap_build_host
   int idxcg_max;
+  // runtime-table path
+  int generic, chem, nel, wselfall;
+  const double* rootpq;    // (J+1)^2
+  const int* jmm;          // idxu_max
+  const int* idxb3;        // idxb_max x 3 (j1, j2, j)
+  const int* idxz_block;   // (J+1)^3
 };
 
 // Per-call device data of one SNAP evaluation.
@@ -189,5 +202,13 @@ void snap_launch_zy32(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const fl
 // FP32 descriptors from the atom-blocked U: Z [N][idxz_max][2] (FP32) and
 // B [N][ncoeff] (FP64 sums).
 void snap_launch_zb32(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const float* U, float* Z, double* B);
+
+// Runtime-table FP64 kernels (ipg_snap_gen.hip): U per element
+// [N][nel][idxu_max][2], Z of the first element pair [N][idxz_max][2],
+// B [N][ncoeff] and the per-pair dB_i/dr_ij [P][ncoeff][3] in the layout of
+// snap_launch_dbdr; eatom (optional) = beta_s(i) . B_i.
+void snap_gen_descriptors(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, double* U, double* Z, double* B,
+                          const double* beta, double* eatom);
+void snap_gen_dbdr(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const double* Z, double* dblist);
 
 }  // namespace ipg

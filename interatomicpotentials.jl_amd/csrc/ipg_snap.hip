@@ -43,13 +43,17 @@ static double clebsch_gordan2(int j1, int j2, int j, int a, int b) {
 void snap_build_host(SnapHost& h, const ipg_snap_params* p) {
   IPG_REQUIRE(p != nullptr, "params is NULL");
   IPG_REQUIRE(p->twojmax >= 0, "twojmax must be >= 0");
-  if (p->twojmax > SNAP_JMAX) fail(IPG_ERR_UNSUPPORTED, "twojmax > 8 is not compiled for the GPU");
+  // init_clebsch_gordan (prebuilt_arrays.jl:257-312) multiplies Int64
+  // factorials; from 2J = 11 on the products overflow and the reference
+  // stops with a DomainError (sqrt of a negative product) or an
+  // OverflowError (factorial(21))
+  if (p->twojmax > SNAP_JMAX_ALL)
+    fail(IPG_ERR_UNSUPPORTED, "twojmax > 10: the reference's Int64 Clebsch-Gordan factorials overflow");
   IPG_REQUIRE(p->nspecies >= 1 && p->nspecies <= 8, "nspecies must be in [1, 8]");
   IPG_REQUIRE(p->radii && p->weight, "radii/weight is NULL");
   // snap.jl:24-27
   IPG_REQUIRE((p->chem_flag != 0) == (p->bnorm_flag != 0),
               "chem_flag and bnorm_flag should be the same, setting bnorm_flag == chem_flag");
-  if (p->chem_flag) fail(IPG_ERR_UNSUPPORTED, "chem_flag = true is not supported on the GPU");
   IPG_REQUIRE(std::isfinite(p->rcutfac) && p->rcutfac > 0, "rcutfac must be finite and > 0");
   IPG_REQUIRE(std::isfinite(p->rfac0) && std::isfinite(p->rmin0), "rfac0/rmin0 must be finite");
   for (int s = 0; s < p->nspecies; s++) {
@@ -95,7 +99,10 @@ void snap_build_host(SnapHost& h, const ipg_snap_params* p) {
       }
   h.idxcg_max = (int)h.cg.size();
   h.idxb_max = (int)h.idxb.size();
-  h.ncoeff = h.idxb_max;  // get_num_snap_coeffs with chem_flag = false
+  h.chem = p->chem_flag != 0;
+  h.nel = h.chem ? p->nspecies : 1;  // prebuilt_arrays.jl:65-69
+  h.generic = h.chem || J > SNAP_JMAX;
+  h.ncoeff = h.idxb_max * h.nel * h.nel * h.nel;  // get_num_snap_coeffs (utl.jl:8-25)
   h.nblk = (int)blocks.size();
   for (int bk = 0; bk < h.nblk; bk++) {
     const int j1 = blocks[bk][0], j2 = blocks[bk][1], j = blocks[bk][2];
@@ -191,6 +198,15 @@ void snap_build_host(SnapHost& h, const ipg_snap_params* p) {
     h.zplan.push_back((uint32_t)z.na | ((uint32_t)z.nb << 4) | ((uint32_t)(z.j1 + 1) << 8) | ((uint32_t)z.j2 << 12) |
                       ((uint32_t)(z.j + 1) << 16));
   }
+  // runtime-table path: sqrt(p/q) (prebuilt_arrays.jl:314-321) and the
+  // (j, mb, ma) of every U entry
+  h.rootpq.assign((size_t)jd * jd, 0.0);
+  for (int pp = 1; pp <= J; pp++)
+    for (int qq = 1; qq <= J; qq++) h.rootpq[pp * jd + qq] = std::sqrt((double)pp / (double)qq);
+  h.jmm.clear();
+  for (int j = 0; j <= J; j++)
+    for (int mb = 0; mb <= j; mb++)
+      for (int ma = 0; ma <= j; ma++) h.jmm.push_back(j | (mb << 8) | (ma << 16));
   h.wself = 1.0;
   h.bzero_j.assign(jd, 0.0);
   if (h.bzero)
@@ -219,10 +235,16 @@ static void snap_upload(ipg_snap* sn) {
     d.radii[s] = s < h.nspecies ? h.radii[s] : 0.0;
     d.weight[s] = s < h.nspecies ? h.weight[s] : 0.0;
   }
-  for (int j = 0; j <= SNAP_JMAX; j++) d.bzero_j[j] = j <= h.J ? h.bzero_j[j] : 0.0;
-  for (int j = 0; j <= SNAP_JMAX + 1; j++) d.idxu_block[j] = j <= h.J + 1 ? h.idxu_block[j] : 0;
+  for (int j = 0; j <= SNAP_JMAX_ALL; j++) d.bzero_j[j] = j <= h.J ? h.bzero_j[j] : 0.0;
+  for (int j = 0; j <= SNAP_JMAX_ALL + 1; j++) d.idxu_block[j] = j <= h.J + 1 ? h.idxu_block[j] : 0;
   d.hoff[0] = 0;
-  for (int j = 0; j <= SNAP_JMAX; j++) d.hoff[j + 1] = d.hoff[j] + (j / 2 + 1) * (j + 1);
+  for (int j = 0; j <= SNAP_JMAX_ALL; j++) d.hoff[j + 1] = d.hoff[j] + (j / 2 + 1) * (j + 1);
+  d.generic = h.generic;
+  d.chem = h.chem;
+  d.nel = h.nel;
+  d.wselfall = h.wselfall;
+  std::vector<int> idxb3;
+  for (const auto& b : h.idxb) idxb3.insert(idxb3.end(), b.begin(), b.end());
   // one allocation for every table
   std::vector<float> cgf(h.cg.begin(), h.cg.end());
   size_t off = 0;
@@ -241,6 +263,10 @@ static void snap_upload(ipg_snap* sn) {
   size_t o_yj = place(sizeof(int) * std::max<size_t>(1, h.yjjz.size()));
   size_t o_yb = place(sizeof(int) * std::max<size_t>(1, h.yjblk.size()));
   size_t o_zp = place(sizeof(uint32_t) * std::max<size_t>(1, h.zplan.size()));
+  size_t o_rp = place(sizeof(double) * h.rootpq.size());
+  size_t o_jm = place(sizeof(int) * h.jmm.size());
+  size_t o_b3 = place(sizeof(int) * idxb3.size());
+  size_t o_zb = place(sizeof(int) * h.idxz_block.size());
   if (hipMalloc(&sn->dmem, off) != hipSuccess) {
     (void)hipGetLastError();
     sn->dmem = nullptr;
@@ -258,6 +284,10 @@ static void snap_upload(ipg_snap* sn) {
   if (!h.yjjz.empty()) h2d(ctx, base + o_yj, h.yjjz.data(), sizeof(int) * h.yjjz.size());
   if (!h.yjblk.empty()) h2d(ctx, base + o_yb, h.yjblk.data(), sizeof(int) * h.yjblk.size());
   h2d(ctx, base + o_zp, h.zplan.data(), sizeof(uint32_t) * h.zplan.size());
+  h2d(ctx, base + o_rp, h.rootpq.data(), sizeof(double) * h.rootpq.size());
+  h2d(ctx, base + o_jm, h.jmm.data(), sizeof(int) * h.jmm.size());
+  h2d(ctx, base + o_b3, idxb3.data(), sizeof(int) * idxb3.size());
+  h2d(ctx, base + o_zb, h.idxz_block.data(), sizeof(int) * h.idxz_block.size());
   sync(ctx);
   d.zent = (const ZEnt*)(base + o_z);
   d.yblk = (const YBlk*)(base + o_y);
@@ -270,6 +300,10 @@ static void snap_upload(ipg_snap* sn) {
   d.yjblk = (const int*)(base + o_yb);
   d.zplan = (const uint32_t*)(base + o_zp);
   d.idxcg_max = h.idxcg_max;
+  d.rootpq = (const double*)(base + o_rp);
+  d.jmm = (const int*)(base + o_jm);
+  d.idxb3 = (const int*)(base + o_b3);
+  d.idxz_block = (const int*)(base + o_zb);
 }
 
 // ---------------------------------------------------------------------------
@@ -321,6 +355,73 @@ __global__ __launch_bounds__(256) void k_force_gather16(int64_t n, const int64_t
     F[3 * i] = fx;
     F[3 * i + 1] = fy;
     F[3 * i + 2] = fz;
+  }
+}
+
+// The last two kernels of the adjoint step over a symmetric CUTOFF list.
+// k_step_forces: the forces, the own row of dE_i/dr_ij minus the reverse
+// pairs rev[q], 16 lanes per atom in a fixed tree (as k_force_gather16<true>,
+// bit for bit), and each block's sum of its atoms' E_i in atom order.
+// k_step_close (one block): the energy beta0 + the block sums in block
+// order (so the sum does not depend on timing) and the step's flag word
+// copied to host-visible memory, so the host needs no device-to-host copy
+// after it.  The kernel boundary orders the two; no device-wide fence.
+constexpr int EPI_ATOMS = 256 / FG_LANES;
+__global__ __launch_bounds__(256) void k_step_forces(int64_t n, const int64_t* __restrict__ off,
+                                                     const int32_t* __restrict__ rev,
+                                                     const double* __restrict__ dedr, double* __restrict__ F,
+                                                     const double* __restrict__ eatom, double* __restrict__ part) {
+  __shared__ double es[EPI_ATOMS];
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = t / FG_LANES;
+  const int l = (int)(t % FG_LANES);
+  double fx = 0, fy = 0, fz = 0;
+  if (i < n) {
+    const int64_t a = off[i], b = off[i + 1];
+    for (int64_t q = a + l; q < b; q += FG_LANES) {
+      fx += dedr[3 * q];
+      fy += dedr[3 * q + 1];
+      fz += dedr[3 * q + 2];
+    }
+    for (int64_t q = a + l; q < b; q += FG_LANES) {
+      const int64_t r = rev[q];
+      fx -= dedr[3 * r];
+      fy -= dedr[3 * r + 1];
+      fz -= dedr[3 * r + 2];
+    }
+  }
+  fx = group_sum<FG_LANES>(fx);
+  fy = group_sum<FG_LANES>(fy);
+  fz = group_sum<FG_LANES>(fz);
+  if (l == 0) {
+    if (i < n) {
+      F[3 * i] = fx;
+      F[3 * i + 1] = fy;
+      F[3 * i + 2] = fz;
+    }
+    es[threadIdx.x / FG_LANES] = i < n ? eatom[i] : 0.0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int k = 0; k < EPI_ATOMS; k++) s += es[k];
+    part[blockIdx.x] = s;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_step_close(int64_t nblocks, const double* __restrict__ part, double beta0,
+                                                    double* __restrict__ energy, const int32_t* __restrict__ flag,
+                                                    int32_t* host_flag) {
+  __shared__ double ws[4];
+  double s = 0.0;
+  for (int64_t b = threadIdx.x; b < nblocks; b += blockDim.x) s += part[b];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double tot = ((ws[0] + ws[1]) + ws[2]) + ws[3];
+    if (energy) *energy = beta0 + tot;
+    *host_flag = *flag;
   }
 }
 
@@ -569,10 +670,17 @@ static SnapCall snap_prepare(ipg_snap* sn, const DevSystem& s, int mode, const i
   return c;
 }
 
-// List bits an energy/force path needs: the adjoint gathers use the
-// reverse-pair index of the symmetric CUTOFF list, the plain path the
-// general reverse rows (its dB gather goes by the center's species).
-static int ef_need(int) { return SNAPL_SYM; }
+// The runtime-table configurations (chem_flag, 2J > 8) run the reference's
+// own route, forces = beta . dB (linear_potential.jl:28-35): the adjoint
+// and FP32 kernels are compiled for 2J <= 8 without chem_flag, and with
+// chem_flag the reference's dB is not the gradient of its B (compute_zi
+// keeps one Z block), so an adjoint would not reproduce its forces.
+static void require_path(const ipg_snap* sn, int path) {
+  IPG_REQUIRE(path == IPG_SNAP_PLAIN_FP64 || path == IPG_SNAP_ADJOINT_FP64 || path == IPG_SNAP_ADJOINT_FP32,
+              "unknown SNAP path");
+  if (sn->host.generic && path != IPG_SNAP_PLAIN_FP64)
+    fail(IPG_ERR_UNSUPPORTED, "chem_flag = true or twojmax > 8: only IPG_SNAP_PLAIN_FP64 (beta . dB) is available");
+}
 
 static void check_species_host(const ipg_snap* sn, const ipg_system* sys, const int32_t* appear) {
   for (int64_t i = 0; i < sys->natoms; i++) {
@@ -611,6 +719,15 @@ static double* run_descriptors_fp64(ipg_snap* sn, const SnapCall& c, double* dB_
   ipg_ctx* ctx = sn->ctx;
   const SnapDev& P = sn->dev;
   const int64_t n = c.R.n;
+  if (P.generic) {  // runtime tables: U per element, Z0, B (always formed: E_i needs it)
+    const size_t n1 = (size_t)std::max<int64_t>(n, 1);
+    double* U = scratch_t<double>(ctx, "snap_U64", 2 * (size_t)P.nel * P.idxu_max * n1);
+    double* Z = scratch_t<double>(ctx, "snap_Z64", 2 * (size_t)P.idxz_max * n1);
+    double* B = B_out ? B_out : scratch_t<double>(ctx, "snap_Bgen", (size_t)P.ncoeff * n1);
+    snap_gen_descriptors(ctx, P, c.R, U, Z, B, beta, eatom);
+    (void)dB_out;
+    return Z;
+  }
   double* U = scratch_t<double>(ctx, "snap_U64", 2 * (size_t)P.idxu_max * std::max<int64_t>(n, 1));
   double* Z = want_z ? scratch_t<double>(ctx, "snap_Z64", 2 * (size_t)P.idxz_max * std::max<int64_t>(n, 1)) : nullptr;
   snap_launch_ui<double>(ctx, P, c.R, U);
@@ -622,7 +739,8 @@ static double* run_descriptors_fp64(ipg_snap* sn, const SnapCall& c, double* dB_
 static double* run_dblist(ipg_snap* sn, const SnapCall& c, const double* Z) {
   const SnapDev& P = sn->dev;
   double* dbl = scratch_t<double>(sn->ctx, "snap_dblist", 3 * (size_t)P.ncoeff * std::max<int64_t>(c.R.npairs, 1));
-  snap_launch_dbdr<double>(sn->ctx, P, c.R, Z, dbl);
+  if (P.generic) snap_gen_dbdr(sn->ctx, P, c.R, Z, dbl);
+  else snap_launch_dbdr<double>(sn->ctx, P, c.R, Z, dbl);
   return dbl;
 }
 
@@ -632,7 +750,7 @@ void snap_gather_db(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const T* d
   if (R.n == 0) return;
   IPG_REQUIRE(R.rev || !Wa, "internal: the fused virial rows need the reverse-pair index");
   // one lane per (coefficient, component), in whole waves
-  const int threads = (3 * P.ncoeff + 63) / 64 * 64;
+  const int threads = std::min(1024, (3 * P.ncoeff + 63) / 64 * 64);
   if (R.rev)
     launch(ctx, "snap_db_gather", [&] {
       k_db_gather_sym<T><<<(unsigned)R.n, threads, 0, ctx->stream>>>(P, R, dbl, dB, pos, Wa);
@@ -654,8 +772,10 @@ static void gather_db(ipg_snap* sn, const SnapCall& c, const double* dbl, double
 // Energy and forces on the device; d_energy/d_forces are device pointers.
 // d_virial (optional, device, 6): beta . W, the virial stress of the
 // linear potential, from the same per-pair derivatives; pos holds the raw
-// positions it needs.
-static void energy_forces_dev(ipg_snap* sn, const SnapCall& c, const double* d_beta, double beta0, int path,
+// positions it needs.  Returns true when the step's last kernel copied the
+// flag word to sn->host_flag (the adjoint paths over a CUTOFF list), so the
+// caller reads it there after synchronizing instead of copying it back.
+static bool energy_forces_dev(ipg_snap* sn, const SnapCall& c, const double* d_beta, double beta0, int path,
                               double* d_energy, double* d_forces, double* d_virial = nullptr,
                               const double* pos = nullptr) {
   ipg_ctx* ctx = sn->ctx;
@@ -668,6 +788,9 @@ static void energy_forces_dev(ipg_snap* sn, const SnapCall& c, const double* d_b
   if (n == 0) {
     zero(ctx, esum, sizeof(double) * 2);
     if (d_virial) zero(ctx, d_virial, sizeof(double) * 6);
+    if (d_energy)
+      launch(ctx, "snap_finish_energy", [&] { k_finish_energy<<<1, 64, 0, ctx->stream>>>(esum, beta0, d_energy); });
+    return false;
   } else if (path == IPG_SNAP_PLAIN_FP64) {
     double* Z = run_descriptors_fp64(sn, c, nullptr, nullptr, eatom, d_beta, true);
     double* dbl = run_dblist(sn, c, Z);
@@ -686,7 +809,7 @@ static void energy_forces_dev(ipg_snap* sn, const SnapCall& c, const double* d_b
       reduce_rows(ctx, Wa, n, 6 * M, Wd, false);
       launch(ctx, "snap_w_beta", [&] { k_w_beta<<<1, 64, 0, ctx->stream>>>(M, Wd, d_beta, d_virial); });
     }
-    return;
+    return false;
   } else {
     double* dedr = scratch_t<double>(ctx, "snap_dedr", 3 * std::max<int64_t>(c.R.npairs, 1));
     if (path == IPG_SNAP_ADJOINT_FP64) {
@@ -707,6 +830,18 @@ static void energy_forces_dev(ipg_snap* sn, const SnapCall& c, const double* d_b
     } else {
       fail(IPG_ERR_INVALID, "unknown SNAP path");
     }
+    if (c.R.rev && !d_virial && sn->host_flag) {
+      // forces, energy and the flag word in two kernels
+      const unsigned gE = grid_for(n * FG_LANES, 256);
+      double* part = scratch_t<double>(ctx, "snap_epart", gE);
+      launch(ctx, "snap_force_gather", [&] {
+        k_step_forces<<<gE, 256, 0, ctx->stream>>>(n, c.R.off, c.R.rev, dedr, F, eatom, part);
+      });
+      launch(ctx, "snap_step_close", [&] {
+        k_step_close<<<1, 256, 0, ctx->stream>>>(gE, part, beta0, d_energy, c.flag, sn->host_flag);
+      });
+      return true;
+    }
     force_gather(ctx, c.R, dedr, F);
     if (d_energy) reduce_rows(ctx, eatom, n, 1, d_energy, false, beta0);  // E = beta0 + sum_i E_i
     if (d_virial) {
@@ -716,10 +851,9 @@ static void energy_forces_dev(ipg_snap* sn, const SnapCall& c, const double* d_b
       });
       reduce_rows(ctx, va, n, 6, d_virial, false);
     }
-    return;
+    return false;
   }
-  if (d_energy)
-    launch(ctx, "snap_finish_energy", [&] { k_finish_energy<<<1, 64, 0, ctx->stream>>>(esum, beta0, d_energy); });
+  return false;
 }
 
 // A CUTOFF-mode list with a pair whose reverse is missing: rebuild the
@@ -848,14 +982,20 @@ static void snap_energy_forces_call(ipg_snap* sn, const DevSystem& s, int mode, 
                                     double* d_forces) {
   ipg_ctx* ctx = sn->ctx;
   SnapGraph& g = sn->graph;
+  require_path(sn, path);
   const bool periodic = s.box.per[0] && s.box.per[1] && s.box.per[2];
   const bool use_graph = ctx->graphs && periodic && s.n > 0 && !g.disabled[ctx->prof ? 1 : 0];
   bool keep_graph = false;
   if (use_graph && graph_matches(g, ctx, s, mode, d_appear, d_beta, beta0, path, d_energy, d_forces)) {
     IPG_HIP(hipGraphLaunch(g.exec, ctx->stream));
     int32_t over = 0;
-    d2h(ctx, &over, g.flag, sizeof(int32_t));
-    sync(ctx);
+    if (g.pinned) {
+      sync(ctx);
+      over = *(volatile int32_t*)sn->host_flag;
+    } else {
+      d2h(ctx, &over, g.flag, sizeof(int32_t));
+      sync(ctx);
+    }
     g.replays++;
     if (ctx->prof) add_graph_prof(ctx, g.events);
     if (over & 16) fail(IPG_ERR_INVALID, "species index out of range of snap.species");
@@ -864,12 +1004,17 @@ static void snap_energy_forces_call(ipg_snap* sn, const DevSystem& s, int mode, 
     keep_graph = (over & 11) == 0;  // only reverse pairs were missing: the capacities hold
   }
   // exact run: the result of this call
-  SnapCall c = snap_prepare(sn, s, mode, d_appear, ef_need(path));
-  energy_forces_dev(sn, c, d_beta, beta0, path, d_energy, d_forces);
+  SnapCall c = snap_prepare(sn, s, mode, d_appear, SNAPL_SYM);
+  const bool pinned = energy_forces_dev(sn, c, d_beta, beta0, path, d_energy, d_forces);
   {
     int32_t f = 0;
-    d2h(ctx, &f, c.flag, sizeof(int32_t));
-    sync(ctx);
+    if (pinned) {
+      sync(ctx);
+      f = *(volatile int32_t*)sn->host_flag;
+    } else {
+      d2h(ctx, &f, c.flag, sizeof(int32_t));
+      sync(ctx);
+    }
     if (f & 16) fail(IPG_ERR_INVALID, "species index out of range of snap.species");
     if (c.R.rev && (f & 4)) fix_asymmetric(sn, c, path, d_beta, d_forces);
   }
@@ -881,12 +1026,14 @@ static void snap_energy_forces_call(ipg_snap* sn, const DevSystem& s, int mode, 
   g.avg_nbr = (double)nb / (double)s.n;
   g.ellK = c.L.ellK;
   g.flag = c.flag;  // cleared by the first launch of every step
+  bool step_pinned = false;
   auto step = [&] {
-    SnapCall cc = snap_prepare(sn, s, mode, d_appear, ef_need(path), g.capA, g.capB, g.ellK);
+    SnapCall cc = snap_prepare(sn, s, mode, d_appear, SNAPL_SYM, g.capA, g.capB, g.ellK);
     cc.R.avg_nbr = g.avg_nbr;
-    energy_forces_dev(sn, cc, d_beta, beta0, path, d_energy, d_forces);
+    step_pinned = energy_forces_dev(sn, cc, d_beta, beta0, path, d_energy, d_forces);
   };
   if (!graph_capture(ctx, g, step)) return;  // this call's result came from the exact run
+  g.pinned = step_pinned;
   g.n = s.n;
   g.pos = s.pos;
   g.species = s.species;
@@ -918,8 +1065,15 @@ int ipg_snap_create(ipg_ctx* ctx, const ipg_snap_params* params, ipg_snap** out)
     try {
       snap_build_host(sn->host, params);
       snap_upload(sn);
+      if (hipHostMalloc((void**)&sn->host_flag, 64, hipHostMallocCoherent) != hipSuccess) {
+        (void)hipGetLastError();
+        sn->host_flag = nullptr;
+        fail(IPG_ERR_NOMEM, "hipHostMalloc of the SNAP status word failed");
+      }
+      *sn->host_flag = 0;
     } catch (...) {
       if (sn->dmem) (void)hipFree(sn->dmem);
+      if (sn->host_flag) (void)hipHostFree(sn->host_flag);
       delete sn;
       throw;
     }
@@ -936,6 +1090,7 @@ int ipg_snap_destroy(ipg_snap* sn) {
     }
     if (sn->dmem) (void)hipFree(sn->dmem);
     if (sn->beta_dev) (void)hipFree(sn->beta_dev);
+    if (sn->host_flag) (void)hipHostFree(sn->host_flag);
     delete sn;
   });
 }
@@ -1075,14 +1230,13 @@ int ipg_snap_energy_forces_virial(ipg_snap* sn, const ipg_system* sys, int mode,
   return guard([&] {
     IPG_REQUIRE(sn && sys, "NULL argument");
     IPG_REQUIRE(std::isfinite(beta0), "beta0 must be finite");
-    IPG_REQUIRE(path == IPG_SNAP_PLAIN_FP64 || path == IPG_SNAP_ADJOINT_FP64 || path == IPG_SNAP_ADJOINT_FP32,
-                "unknown SNAP path");
+    require_path(sn, path);
     ipg_ctx* ctx = sn->ctx;
     IPG_HIP(hipSetDevice(ctx->device));
     check_species_host(sn, sys, appear);
     const double* db = expand_beta(sn, beta, nbeta);
     DevSystem s = upload_system(ctx, sys, "snap_api");
-    SnapCall c = snap_prepare(sn, s, mode, upload_appear(ctx, appear, s.n), ef_need(path));
+    SnapCall c = snap_prepare(sn, s, mode, upload_appear(ctx, appear, s.n), SNAPL_SYM);
     double* out = scratch_t<double>(ctx, "snap_api_vout", 8 + 3 * std::max<int64_t>(s.n, 1));
     energy_forces_dev(sn, c, db, beta0, path, out, out + 8, out + 1, s.pos);
     int32_t f = 0;
@@ -1122,6 +1276,8 @@ int ipg_snap_force_vjp(ipg_snap* sn, const ipg_system* sys, int mode, const int3
   return guard([&] {
     IPG_REQUIRE(sn && sys && gF, "NULL argument");
     IPG_REQUIRE(precision == IPG_FP64 || precision == IPG_FP32, "precision must be IPG_FP64 or IPG_FP32");
+    if (sn->host.generic)
+      fail(IPG_ERR_UNSUPPORTED, "chem_flag = true or twojmax > 8: the force pullback is compiled for 2J <= 8");
     ipg_ctx* ctx = sn->ctx;
     IPG_HIP(hipSetDevice(ctx->device));
     check_species_host(sn, sys, appear);

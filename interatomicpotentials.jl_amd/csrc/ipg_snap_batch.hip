@@ -131,7 +131,12 @@ static void per_pair(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const Bat
                      const BatchOut& o, const char* name) {
   if (!o.dB && !o.wrow) return;
   T* dbl = scratch_t<T>(ctx, name, 3 * (size_t)P.ncoeff * std::max<int64_t>(R.npairs, 1));
-  snap_launch_dbdr<T>(ctx, P, R, Z, dbl);
+  if constexpr (std::is_same<T, double>::value) {
+    if (P.generic) snap_gen_dbdr(ctx, P, R, Z, dbl);
+    else snap_launch_dbdr<T>(ctx, P, R, Z, dbl);
+  } else {
+    snap_launch_dbdr<T>(ctx, P, R, Z, dbl);
+  }
   double* Wa = o.wrow ? scratch_t<double>(ctx, "batch_Wa", 6 * (size_t)P.ncoeff * std::max<int64_t>(b.n, 1)) : nullptr;
   if (R.rev) {
     snap_gather_db<T>(ctx, P, R, dbl, o.dB, b.pos, Wa);
@@ -186,6 +191,11 @@ static SnapListDev batch_run(ipg_snap* sn, const BatchSys& b, int fp32, const Ba
     snap_launch_ui<float>(ctx, P, R, U);
     snap_launch_zb32(ctx, P, R, U, Z, Bv);
     per_pair<float>(ctx, P, R, b, Z, o, "batch_dbl32");
+  } else if (P.generic) {  // runtime tables (chem_flag, 2J > 8)
+    double* U = scratch_t<double>(ctx, "batch_U64", 2 * (size_t)P.nel * P.idxu_max * n);
+    double* Z = scratch_t<double>(ctx, "batch_Z64", 2 * (size_t)P.idxz_max * n);
+    snap_gen_descriptors(ctx, P, R, U, Z, Bv, nullptr, nullptr);
+    per_pair<double>(ctx, P, R, b, Z, o, "batch_dbl64");
   } else {
     double* U = scratch_t<double>(ctx, "batch_U64", 2 * (size_t)P.idxu_max * n);
     double* Z = scratch_t<double>(ctx, "batch_Z64", 2 * (size_t)P.idxz_max * n);
@@ -217,6 +227,11 @@ static void batch_tune(ipg_snap* sn, const BatchSys& b, int fp32, BatchTune& tn,
     if (tn.ms_scan[k] < tn.ms_scan[best]) best = k;
   }
   tn.scan_lp = lps[best];
+  if (P.generic) {  // the runtime-table kernels have one launch shape
+    tn.ui_apw = 1;
+    tn.dbdr_threads = 64;
+    return;
+  }
   zero(ctx, flag, sizeof(int32_t));
   SnapListDev L = build_snap_list_batch(ctx, b, cut, SNAPL_SYM, "snap_batch", 0, flag, 0, tn.scan_lp);
   SnapRun R;
@@ -339,7 +354,9 @@ static BatchSys batch_layout(ipg_snap* sn, int64_t nconf, const int64_t* natoms,
 static void batch_descriptors(ipg_snap* sn, const BatchSys& b, int precision, const BatchOut& o) {
   ipg_ctx* ctx = sn->ctx;
   IPG_REQUIRE(precision == IPG_FP32 || precision == IPG_FP64, "precision must be IPG_FP64 or IPG_FP32");
-  IPG_REQUIRE(!sn->host.bnorm, "bnorm is not supported by the batched descriptors");
+  // chem_flag (and with it bnorm_flag) and 2J > 8 run the runtime-table FP64 kernels
+  if (sn->host.generic && precision != IPG_FP64)
+    fail(IPG_ERR_UNSUPPORTED, "chem_flag = true or twojmax > 8: the batched descriptors run in FP64 only");
   const int fp32 = precision == IPG_FP32 ? 1 : 0;
   int32_t* flag = scratch_t<int32_t>(ctx, "batch_flag", 1);
   BatchTune& tn = sn->btune[fp32];

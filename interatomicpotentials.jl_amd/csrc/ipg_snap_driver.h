@@ -31,7 +31,8 @@ struct SnapGraph {
   int64_t capA = 0, capB = 0;
   int ellK = 0;  // neighbor row width of the CUTOFF list's first pass
   double avg_nbr = -1.0;
-  int32_t* flag = nullptr;  // bits of the last replay: 1, 2, 8 capacity; 4 missing reverse pair
+  int32_t* flag = nullptr;  // bits of the last replay: 1, 2, 8 capacity; 4 missing reverse pair; 16 species
+  bool pinned = false;      // the step's last kernel copies the flag to the snap object's host word
   std::vector<ipg_ctx::Pending> events;  // profiling event pairs inside the graph
   bool disabled[2] = {false, false};     // capture failed (without / with profiling events)
   std::string reason;
@@ -77,4 +78,5 @@ struct ipg_snap {
   ipg::BatchLayout blayout;
   int bprec = -1;              // precision the captured batch pipeline was built for
   int64_t bpairs = 0;          // SNAP pairs of the last exact batched run
+  int32_t* host_flag = nullptr;  // pinned host word: the flag of the last step that ran the epilogue
 };

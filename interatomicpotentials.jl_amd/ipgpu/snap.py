@@ -54,6 +54,8 @@ class SNAP:
         self.weight = np.ascontiguousarray(weight, dtype=np.float64)
         self.chem_flag, self.bzero_flag, self.bnorm_flag = bool(chem_flag), bool(bzero_flag), bool(bnorm_flag)
         self.switch_flag, self.wselfall_flag = bool(switch_flag), bool(wselfall_flag)
+        # runtime-table kernels (FP64, beta . dB route only)
+        self.generic = self.chem_flag or self.twojmax > 8
         if list_mode not in _MODES:
             raise ValueError("list_mode must be 'reference' or 'cutoff'")
         self.list_mode = list_mode
@@ -252,21 +254,31 @@ def force_from_descriptors(dB, lbp):
     return np.einsum("nkm,m->nk", np.asarray(dB), lbp.expanded_beta())
 
 
-def energy_and_force(s, lbp, path="adjoint", list_mode=None):
+def _path(snap, path):
+    """Default path: 'adjoint', or 'plain' for the configurations that only
+    have the reference's beta . dB route (chem_flag, twojmax > 8)."""
+    if path is None:
+        return "plain" if snap.generic else "adjoint"
+    return path
+
+
+def energy_and_force(s, lbp, path=None, list_mode=None):
     """basis_potentials.jl:32-41.  path: 'plain' (FP64 dB then beta, as the
-    reference), 'adjoint' (FP64), 'adjoint_fp32' (FP32, FP64 reductions)."""
+    reference), 'adjoint' (FP64), 'adjoint_fp32' (FP32, FP64 reductions);
+    None: 'adjoint', or 'plain' where that is the only route (chem_flag,
+    twojmax > 8)."""
     L = _lib.load()
     snap = lbp.basis
     cs, keep, appear = snap._sys(s)
     e = C.c_double()
     f = np.zeros((cs.natoms, 3))
     check(L.ipg_snap_energy_forces(snap._h, C.byref(cs), snap._mode(list_mode), appear.ctypes.data,
-                                   lbp.beta.ctypes.data, lbp.beta.size, lbp.beta0, _PATHS[path], C.byref(e),
+                                   lbp.beta.ctypes.data, lbp.beta.size, lbp.beta0, _PATHS[_path(snap, path)], C.byref(e),
                                    f.ctypes.data))
     return EnergyForce(e.value, f)
 
 
-def virial_stress(s, lbp, path="adjoint", list_mode=None):
+def virial_stress(s, lbp, path=None, list_mode=None):
     """basis_potentials.jl:43-50 with linear_potential.jl:37-42: W * beta,
     W the virial descriptors of compute_all_descriptors (snap.jl:267-281,
     raw positions), rows [xx, yy, zz, zy, zx, yx].  path 'plain' forms W
@@ -277,7 +289,7 @@ def virial_stress(s, lbp, path="adjoint", list_mode=None):
     cs, keep, appear = snap._sys(s)
     v = np.zeros(6)
     check(L.ipg_snap_virial_stress(snap._h, C.byref(cs), snap._mode(list_mode), appear.ctypes.data,
-                                   lbp.beta.ctypes.data, lbp.beta.size, _PATHS[path], v.ctypes.data))
+                                   lbp.beta.ctypes.data, lbp.beta.size, _PATHS[_path(snap, path)], v.ctypes.data))
     return v
 
 
@@ -289,7 +301,7 @@ def virial(s, lbp, **kw):
 EnergyForceVirial = collections.namedtuple("EnergyForceVirial", ["e", "f", "v"])
 
 
-def energy_force_virial(s, lbp, path="adjoint", list_mode=None):
+def energy_force_virial(s, lbp, path=None, list_mode=None):
     """Energy, forces and virial stress from one evaluation."""
     L = _lib.load()
     snap = lbp.basis
@@ -298,7 +310,7 @@ def energy_force_virial(s, lbp, path="adjoint", list_mode=None):
     f = np.zeros((cs.natoms, 3))
     v = np.zeros(6)
     check(L.ipg_snap_energy_forces_virial(snap._h, C.byref(cs), snap._mode(list_mode), appear.ctypes.data,
-                                          lbp.beta.ctypes.data, lbp.beta.size, lbp.beta0, _PATHS[path],
+                                          lbp.beta.ctypes.data, lbp.beta.size, lbp.beta0, _PATHS[_path(snap, path)],
                                           C.byref(e), f.ctypes.data, v.ctypes.data))
     return EnergyForceVirial(e.value, f, v)
 
@@ -326,7 +338,7 @@ def force_vjp(s, lbp, gF, precision="fp64", list_mode=None):
     return gb, gp
 
 
-def force_rrule(s, lbp, path="adjoint", precision="fp64", list_mode=None):
+def force_rrule(s, lbp, path=None, precision="fp64", list_mode=None):
     """ChainRulesCore-style rrule of force(s, lbp): returns (F, pullback),
     pullback(gF) -> (gbeta, gpositions)."""
     F = energy_and_force(s, lbp, path=path, list_mode=list_mode).f

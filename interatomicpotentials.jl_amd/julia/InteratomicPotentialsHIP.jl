@@ -276,14 +276,18 @@ function snap_energy_forces(s, lbp, path; mode=IPG_SNAP_LIST_REFERENCE)
     end
 end
 
-function InteratomicPotentials.energy_and_force(s::AbstractSystem, h::HIP{<:LBasisPotential}; path=IPG_SNAP_ADJOINT_FP64)
+# chem_flag = true and twojmax > 8 have only the reference's β . dB route on the GPU
+default_path(snap) = (snap.chem_flag || snap.twojmax > 8) ? IPG_SNAP_PLAIN_FP64 : IPG_SNAP_ADJOINT_FP64
+
+function InteratomicPotentials.energy_and_force(s::AbstractSystem, h::HIP{<:LBasisPotential};
+                                                path=default_path(h.p.basis))
     e, f = snap_energy_forces(s, h.p, path)
     (; e, f)
 end
 
 # basis_potentials.jl:43-50 with linear_potential.jl:37-42: SVector{6}(W * β),
 # W the virial descriptors of compute_all_descriptors (snap.jl:267-281).
-function InteratomicPotentials.virial_stress(s::AbstractSystem, h::HIP{<:LBasisPotential}; path=IPG_SNAP_ADJOINT_FP64,
+function InteratomicPotentials.virial_stress(s::AbstractSystem, h::HIP{<:LBasisPotential}; path=default_path(h.p.basis),
                                              mode=IPG_SNAP_LIST_REFERENCE)
     lbp = h.p
     snap = lbp.basis

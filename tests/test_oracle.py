@@ -557,3 +557,222 @@ def test_threaded_pair_path_equals_serial(orc):
         assert e == pytest.approx(e0, rel=1e-12)
         np.testing.assert_allclose(f, f0, rtol=0, atol=1e-12 * np.abs(f0).max())
         np.testing.assert_allclose(v, v0, rtol=0, atol=1e-12 * np.abs(v0).max())
+
+
+# ------------------------------------- chem_flag and 2J = 9, 10 pins ----
+def _one_element(orc, seed, J, chem):
+    rng = np.random.default_rng(seed)
+    b = np.array([[0, 0, 0], [0.5, 0.5, 0.5]])
+    g = np.array(list(np.ndindex(3, 3, 3)))
+    X = ((g[:, None] + b[None]) * 3.16).reshape(-1, 3) + rng.normal(scale=0.05, size=(54, 3))
+    L = np.full(3, 9.48)
+    s = orc.Snap(J, 1, 2.5, 0.0, 0.99363, [1.0], [1.0], chem_flag=chem, bnorm_flag=chem, switch_flag=True)
+    return s, X, L, np.zeros(54, dtype=np.int32)
+
+
+@pytest.mark.parametrize("J", [3, 6])
+def test_chem_single_element_is_plain_over_jplus1(orc, J):
+    """With one element, chem_flag = true (bnorm forced on) divides each Z
+    block by its own j+1 once (compute_z.jl:53-56): B(j1, j2, j) is the plain
+    B / (j+1).  dB/dr too: its three half-sums use the blocks (j1, j2, j),
+    (j, j2, j1), (j, j1, j2), now divided by j+1, j1+1, j2+1, where the plain
+    path multiplies the last two by (j+1)/(j1+1) and (j+1)/(j2+1)
+    (compute_b.jl:268-270, 333-335).  W is linear in dB.  A j1 or j2 read
+    for j in either factor fails this."""
+    s0, X, L, sp = _one_element(orc, 9, J, False)
+    s1, _, _, _ = _one_element(orc, 9, J, True)
+    off, nb, R, r = s0.neighborlist(X, L, sp, mode=orc.SNAP_LIST_CUTOFF)
+    B0, dB0, W0 = s0.all_descriptors(X, sp, off, nb, r)
+    B1, dB1, W1 = s1.all_descriptors(X, sp, off, nb, r)
+    jf = s0.idxb()[:, 2] + 1.0
+    np.testing.assert_allclose(B1, B0 / jf, rtol=1e-13, atol=0)
+    np.testing.assert_allclose(dB1, dB0 / jf[None, :, None], rtol=0, atol=1e-13 * np.abs(dB0).max())
+    np.testing.assert_allclose(W1, W0 / jf[:, None], rtol=0, atol=1e-13 * np.abs(W0).max())
+
+
+class _CG:
+    """Dense Clebsch-Gordan coefficients from sympy, cached (arguments doubled)."""
+
+    def __init__(self):
+        self.c = {}
+
+    def __call__(self, j1, a, j2, b, j, m):
+        k = (j1, a, j2, b, j, m)
+        if k not in self.c:
+            from sympy import S
+            from sympy.physics.wigner import clebsch_gordan
+
+            self.c[k] = 0.0 if (abs(a + b) > j or a + b != m) else float(
+                clebsch_gordan(S(j1) / 2, S(j2) / 2, S(j) / 2, S(a) / 2, S(b) / 2, S(m) / 2))
+        return self.c[k]
+
+
+def _brute_Z(U1, U2, j1, j2, j, C):
+    """Z(j1, j2, j)[mb, ma] from the definition, full (j+1) x (j+1)."""
+    Z = np.zeros((j + 1, j + 1), complex)
+    for mb in range(j + 1):
+        for ma in range(j + 1):
+            z = 0.0
+            for mb1 in range(j1 + 1):
+                mb2 = mb - mb1 + (j1 + j2 - j) // 2
+                if not 0 <= mb2 <= j2:
+                    continue
+                cb = C(j1, 2 * mb1 - j1, j2, 2 * mb2 - j2, j, 2 * mb - j)
+                if cb == 0:
+                    continue
+                for ma1 in range(j1 + 1):
+                    ma2 = ma - ma1 + (j1 + j2 - j) // 2
+                    if not 0 <= ma2 <= j2:
+                        continue
+                    z += cb * C(j1, 2 * ma1 - j1, j2, 2 * ma2 - j2, j, 2 * ma - j) * U1[j1][mb1, ma1] * U2[j2][mb2, ma2]
+            Z[mb, ma] = z
+    return Z
+
+
+def test_chem_mixed_elements_brute_force(orc):
+    """chem_flag = true with two elements on a 5-atom open cluster, from
+    the definitions: per-element U sums (U from the closed-form-pinned
+    uarray, wself on the center's element, compute_ui.jl:131-175), Z of
+    every element pair by dense sympy CG sums, the as-is Z0 chain
+    (compute_z.jl:46-56: every pair added to block 0, divided by j+1 after
+    each), B of the triples (0, 0, e3) as full-matrix sums (twice the
+    half-sum), all other triples zero; dB/dr from central differences of
+    sfac w U of the pair contracted with the frozen Z0 at the three levels
+    (compute_b.jl:145-337) into the triples (0, 0, e3), (e3, 0, 0),
+    (0, e3, 0) of the neighbor's element e3, gathered with +/- as
+    snap.jl:264-265.  Swapping two placements or dropping the chain's
+    division fails it."""
+    J, nel = 3, 2
+    C = _CG()
+    rng = np.random.default_rng(21)
+    X = rng.uniform(-1.2, 1.2, size=(5, 3))
+    L = np.array([INF, INF, INF])
+    sp = np.array([0, 1, 1, 0, 1], dtype=np.int32)
+    radii, weight, rfac, rcutfac = [1.1, 1.3], [1.0, 0.7], 0.99363, 1.4
+    s = orc.Snap(J, nel, rcutfac, 0.0, rfac, radii, weight, chem_flag=True, bnorm_flag=True, switch_flag=True)
+    off, nb, R, r = s.neighborlist(X, L, sp, mode=orc.SNAP_LIST_CUTOFF)
+    B, dB, W = s.all_descriptors(X, sp, off, nb, r)
+    assert off[-1] > 10
+    ub, idxb, nb_ = s.idxu_block(), s.idxb(), s.idxb_max
+    ntr = nel ** 3
+
+    def sfac(rr, rc):
+        return 0.5 * (np.cos(rr * np.pi / rc) + 1.0)
+
+    def U_of(r3, rc, w):
+        u = s.uarray(r3, rc)
+        rr = np.linalg.norm(r3)
+        return {j: sfac(rr, rc) * w * u[ub[j]:ub[j] + (j + 1) ** 2].reshape(j + 1, j + 1) for j in range(J + 1)}
+
+    n = len(sp)
+    Bb = np.zeros((n, ntr, nb_))
+    dBb = np.zeros((n, nel, ntr, nb_, 3))
+    for i in range(n):
+        si = sp[i]
+        Ue = [{j: (np.eye(j + 1, dtype=complex) if e == si else np.zeros((j + 1, j + 1), complex))
+               for j in range(J + 1)} for e in range(nel)]
+        pairs = []
+        for q in range(off[i], off[i + 1]):
+            jn = nb[q]
+            rc = (radii[si] + radii[sp[jn]]) * rcutfac
+            u = U_of(r[q], rc, weight[sp[jn]])
+            for j in range(J + 1):
+                Ue[sp[jn]][j] = Ue[sp[jn]][j] + u[j]
+            pairs.append((jn, r[q], rc))
+        Z0 = {}
+
+        def z0(j1, j2, j):
+            if (j1, j2, j) not in Z0:
+                z = np.zeros((j + 1, j + 1), complex)
+                for e1 in range(nel):
+                    for e2 in range(nel):
+                        z = (z + _brute_Z(Ue[e1], Ue[e2], j1, j2, j, C)) / (j + 1)
+                Z0[(j1, j2, j)] = z
+            return Z0[(j1, j2, j)]
+
+        for k, (j1, j2, j) in enumerate(idxb):
+            for e3 in range(nel):
+                Bb[i, e3, k] = (np.conj(Ue[e3][j]) * z0(j1, j2, j)).real.sum()
+        h = 1e-6
+        for (jn, r3, rc) in pairs:
+            e3 = sp[jn]
+            for kk in range(3):
+                rp, rm = r3.copy(), r3.copy()
+                rp[kk] += h
+                rm[kk] -= h
+                up, um = U_of(rp, rc, weight[e3]), U_of(rm, rc, weight[e3])
+                du = {j: (up[j] - um[j]) / (2 * h) for j in range(J + 1)}
+                for k, (j1, j2, j) in enumerate(idxb):
+                    t1 = (np.conj(du[j]) * z0(j1, j2, j)).real.sum()
+                    t2 = (np.conj(du[j1]) * z0(j, j2, j1)).real.sum()
+                    t3 = (np.conj(du[j2]) * z0(j, j1, j2)).real.sum()
+                    row = np.zeros(ntr)
+                    row[e3] += t1
+                    row[e3 * nel * nel] += t2
+                    row[e3 * nel] += t3
+                    dBb[i, si, :, k, kk] += row
+                    dBb[jn, si, :, k, kk] -= row
+    np.testing.assert_allclose(B, Bb.reshape(n, -1), rtol=0, atol=1e-12 * np.abs(Bb).max())
+    dBr = dBb.reshape(n, nel * ntr * nb_, 3)
+    assert np.abs(dBr).max() > 0
+    np.testing.assert_allclose(dB, dBr, rtol=0, atol=1e-7 * np.abs(dBr).max())
+
+
+def test_high_levels_tables_and_wigner(orc):
+    """2J = 9, 10 (the largest twojmax the reference's Int64 Clebsch-Gordan
+    factorials support): closed-form table sizes, exact CG on the top
+    levels against sympy, and U against the closed-form Wigner matrices."""
+    from sympy import S
+    from sympy.physics.wigner import clebsch_gordan
+
+    for J in (9, 10):
+        s = orc.Snap(J, 1, 1.0, 0.0, 0.99363, [1.0], [1.0])
+        assert s.idxu_max == sum((j + 1) ** 2 for j in range(J + 1))
+        m = J // 2 + 1 if J % 2 == 0 else (J + 1) // 2
+        assert s.ncoeff == s.idxb_max == (m * (m + 1) * (2 * m + 1) // 6 if J % 2 == 0 else m * (m + 1) * (m + 2) // 3)
+    J = 10
+    s = orc.Snap(J, 1, 1.0, 0.0, 0.99363, [1.0], [1.0])
+    cg, blk = s.cglist(), s.idxcg_block()
+    err = 0.0
+    for (j1, j2, j) in [(10, 10, 10), (10, 9, 9), (10, 6, 6), (9, 8, 3), (10, 10, 0)]:
+        for m1 in range(j1 + 1):
+            for m2 in range(j2 + 1):
+                a, b = 2 * m1 - j1, 2 * m2 - j2
+                ex = 0.0 if abs(a + b) > j else float(
+                    clebsch_gordan(S(j1) / 2, S(j2) / 2, S(j) / 2, S(a) / 2, S(b) / 2, S(a + b) / 2))
+                err = max(err, abs(cg[blk[j1, j2, j] + m1 * (j2 + 1) + m2] - ex))
+    assert err < 1e-13
+    ub = s.idxu_block()
+    rng = np.random.default_rng(15)
+    for _ in range(3):
+        r3 = rng.normal(size=3)
+        r3 *= rng.uniform(0.3, 1.9) / np.linalg.norm(r3)
+        U = s.uarray(r3, 2.0)
+        r = np.linalg.norm(r3)
+        z0 = r / np.tan(r * 0.99363 * np.pi / 2.0)
+        r0 = np.sqrt(r * r + z0 * z0)
+        a, b = (z0 - 1j * r3[2]) / r0, (r3[1] - 1j * r3[0]) / r0
+        for j in (9, 10):
+            M = U[ub[j]:ub[j] + (j + 1) ** 2].reshape(j + 1, j + 1)
+            assert np.abs(M - wigner_D_closed(a, b, j)).max() < 1e-11, j
+
+
+def test_high_levels_dB_finite_differences(orc):
+    """dB/dr at 2J = 10 against central differences of sum B (as
+    test_dB_finite_differences, at the top level the reference supports)."""
+    s, X, L, sp = _bcc(orc, 5, True, J=10)
+
+    def Btot(Xc):
+        off, nb, R, r = s.neighborlist(Xc, L, sp, mode=orc.SNAP_LIST_CUTOFF)
+        B = s.local_descriptors(sp, off, nb, r)
+        return np.concatenate([B[sp == t].sum(0) for t in (0, 1)])
+
+    off, nb, R, r = s.neighborlist(X, L, sp, mode=orc.SNAP_LIST_CUTOFF)
+    B, dB, W = s.all_descriptors(X, sp, off, nb, r)
+    h = 1e-5
+    for (a, k) in [(3, 0), (40, 2)]:
+        Xp, Xm = X.copy(), X.copy()
+        Xp[a, k] += h
+        Xm[a, k] -= h
+        fd = -(Btot(Xp) - Btot(Xm)) / (2 * h)
+        assert np.abs(fd - dB[a, :, k]).max() < 1e-6 * np.abs(dB[a, :, k]).max()
