@@ -5,6 +5,7 @@
 // Clebsch-Gordan coefficients from the Racah closed form); the derived
 // tables (Y coefficients, dB half-sum terms) are this engine's own.
 #include "ipg_snap_driver.h"
+#include "ipg_snap_f32.h"
 #include "ipg_snap_kernels.h"
 
 #include <algorithm>
@@ -824,9 +825,9 @@ static bool energy_forces_dev(ipg_snap* sn, const SnapCall& c, const double* d_b
       const size_t nblk32 = (size_t)((n + 31) / 32) * 32;  // atom-blocked layout, padded to whole batches
       float* U = scratch_t<float>(ctx, "snap_U32", 2 * (size_t)P.idxu_max * nblk32);
       float* Y = scratch_t<float>(ctx, "snap_Y32", 2 * (size_t)P.idxu_max * nblk32);
-      snap_launch_ui<float>(ctx, P, c.R, U);
+      snap_launch_ui_rows32(ctx, P, c.R, U);
       snap_launch_zy32(ctx, P, c.R, U, d_beta, Y, eatom);
-      snap_launch_dedr<float>(ctx, P, c.R, Y, dedr, nullptr);
+      snap_launch_dedr_rev32(ctx, P, c.R, Y, dedr);
     } else {
       fail(IPG_ERR_INVALID, "unknown SNAP path");
     }

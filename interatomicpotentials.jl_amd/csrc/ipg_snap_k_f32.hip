@@ -1,7 +1,8 @@
-// ipg_snap_k_f32.hip -- SNAP kernel instantiations for S = float (U and
-// force kernels on the atom-blocked layout; Z/B/Y is ipg_snap_zy.hip).
+// ipg_snap_k_f32.hip -- SNAP kernel instantiation for S = float: the
+// neighbor-parallel U kernel of the batched descriptors (the single-system
+// FP32 U and force kernels are in ipg_snap_f32.hip, Z/B/Y in ipg_snap_zy.hip).
 #include "ipg_snap_kernels.h"
 
 namespace ipg {
-IPG_SNAP_INSTANTIATE_UD(float)
+IPG_SNAP_INSTANTIATE_U(float)
 }  // namespace ipg

@@ -14,7 +14,6 @@
 // reference's copy loop at :103-125 does exactly that).
 #pragma once
 #include "ipg_snap.h"
-#include "ipg_wigner_adj.h"
 
 #include <algorithm>
 
@@ -789,74 +788,6 @@ __global__ __launch_bounds__(256) void k_snap_dedr(SnapDev P, SnapRun R, const S
   }
 }
 
-// ---------------------------------------------------------------------------
-// K3 in FP32: the same dE_i/dr_ij by the reverse-mode sweep of
-// ipg_wigner_adj.h.  Phi = 2 sum_half w(e) Re(conj(u_e) Y_i,e) and its
-// adjoints with respect to the Cayley-Klein parameters (a, b) come from one
-// backward pass through the recursion (no du/dr rows); then
-//   dE_i/dr_ij = dsfac r_hat Phi + sfac grad_r Phi,
-// with grad_r Phi = Re(a_bar) da_r + Im(a_bar) da_i + Re(b_bar) db_r +
-// Im(b_bar) db_i from the FP64 Jacobian of (a, b).  One thread per pair.
-// ---------------------------------------------------------------------------
-// Seeds of the sweep: Y_i,e in the atom-blocked layout.  order() launders a
-// zero offset through an empty asm that depends on x, so a level's reads
-// cannot be hoisted above the point where the sweep reaches it; tie() does
-// the same for a value, so a row's recursion starts after the previous row.
-struct YSeed {
-  const float2* y;
-  __device__ __forceinline__ int order(float x) const {
-    int z = 0;
-    asm volatile("; order %1" : "+v"(z) : "v"(x));
-    return z;
-  }
-  __device__ __forceinline__ float tie(float v, float x) const {
-    asm volatile("; tie %1" : "+v"(v) : "v"(x));
-    return v;
-  }
-  template <int jl, int mb, int ma>
-  __device__ __forceinline__ float2 operator()(IC<jl>, IC<mb>, IC<ma>, int tok) const {
-    return y[32 * (ub_of(jl) + (jl + 1) * mb + ma) + tok];
-  }
-};
-
-template <int J>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_snap_dedr_rev32(SnapDev P, SnapRun R, const float* __restrict__ Y,
-                                                         double* __restrict__ dedr) {
-  constexpr int IDXU = ub_of(J + 1);
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= (R.d_np ? *R.d_np : R.npairs)) return;
-  const int32_t i = R.center[q], j = R.nbr[q];
-  const int si = snap_species(R.species, i, P.nspecies), sj = snap_species(R.species, j, P.nspecies);
-  const double x = R.rvec[3 * q], y = R.rvec[3 * q + 1], z = R.rvec[3 * q + 2];
-  const double rcut = (P.radii[si] + P.radii[sj]) * P.rcutfac;
-  CK<double> d;
-  cayley_klein_impl<double, true>(x, y, z, rcut, P.weight[sj], P, d);
-  const wadj::AB<float> c{(float)d.a_r, (float)d.a_i, (float)d.b_r, (float)d.b_i};
-  // the FP64 Jacobian rows wait in LDS while the sweep runs:
-  // [k][thread] for k = (d a_r, d a_i, d b_r, d b_i) x 3, r_hat x 3, sfac, dsfac
-  __shared__ double jac[17][256];
-  const int t = threadIdx.x;
-#pragma unroll
-  for (int k = 0; k < 3; k++) {
-    jac[k][t] = d.da_r[k];
-    jac[3 + k][t] = d.da_i[k];
-    jac[6 + k][t] = d.db_r[k];
-    jac[9 + k][t] = d.db_i[k];
-    jac[12 + k][t] = d.u[k];
-  }
-  jac[15][t] = d.sfac;
-  jac[16][t] = d.dsfac;
-  // the atom-blocked Y of the Z/Y kernel: [i/32][jju][i%32]
-  YSeed seed{reinterpret_cast<const float2*>(Y) + ((int64_t)(i >> 5) * IDXU * 32 + (i & 31))};
-  wadj::Acc acc;
-  wadj::vjp<float, J>(c, seed, acc);
-  const double g0 = jac[16][t] * acc.phi, sf = jac[15][t];
-#pragma unroll
-  for (int k = 0; k < 3; k++)
-    dedr[3 * q + k] = g0 * jac[12 + k][t] + sf * (acc.ar * jac[k][t] + acc.ai * jac[3 + k][t] +
-                                                  acc.br * jac[6 + k][t] + acc.bi * jac[9 + k][t]);
-}
-
 template <typename T>
 __global__ void k_snap_ycoef(SnapDev P, const double* __restrict__ beta, T* __restrict__ ycoef) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -928,22 +859,6 @@ void snap_launch_dedr(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const S*
   if (R.npairs == 0) return;
   unsigned g = grid_for(R.npairs, 256);
   const char* nm = IsDual<S>::value ? "snap_dedr_dual" : (sizeof(S) == 4 ? "snap_dedr_fp32" : "snap_dedr_fp64");
-  if constexpr (std::is_same<S, float>::value) {
-    launch(ctx, nm, [&] {
-      switch (P.J) {
-        case 0: k_snap_dedr_rev32<0><<<g, 256, 0, ctx->stream>>>(P, R, Y, dedr); break;
-        case 1: k_snap_dedr_rev32<1><<<g, 256, 0, ctx->stream>>>(P, R, Y, dedr); break;
-        case 2: k_snap_dedr_rev32<2><<<g, 256, 0, ctx->stream>>>(P, R, Y, dedr); break;
-        case 3: k_snap_dedr_rev32<3><<<g, 256, 0, ctx->stream>>>(P, R, Y, dedr); break;
-        case 4: k_snap_dedr_rev32<4><<<g, 256, 0, ctx->stream>>>(P, R, Y, dedr); break;
-        case 5: k_snap_dedr_rev32<5><<<g, 256, 0, ctx->stream>>>(P, R, Y, dedr); break;
-        case 6: k_snap_dedr_rev32<6><<<g, 256, 0, ctx->stream>>>(P, R, Y, dedr); break;
-        case 7: k_snap_dedr_rev32<7><<<g, 256, 0, ctx->stream>>>(P, R, Y, dedr); break;
-        default: k_snap_dedr_rev32<8><<<g, 256, 0, ctx->stream>>>(P, R, Y, dedr); break;
-      }
-    });
-    return;
-  }
   launch(ctx, nm, [&] {
     switch (P.J) {
       case 0: k_snap_dedr<0, S><<<g, 256, 0, ctx->stream>>>(P, R, Y, dedr, dedr_t); break;
@@ -959,6 +874,7 @@ void snap_launch_dedr(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const S*
   });
 }
 
+#define IPG_SNAP_INSTANTIATE_U(S) template void snap_launch_ui<S>(ipg_ctx*, const SnapDev&, const SnapRun&, S*);
 #define IPG_SNAP_INSTANTIATE_UD(S)                                                                               \
   template void snap_launch_ui<S>(ipg_ctx*, const SnapDev&, const SnapRun&, S*);                                 \
   template void snap_launch_dedr<S>(ipg_ctx*, const SnapDev&, const SnapRun&, const S*, double*, double*);
@@ -976,7 +892,7 @@ void snap_launch_dedr(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const S*
   IPG_SNAP_EXTERN_UD(S)                                                                                          \
   extern template void snap_launch_zby<S>(ipg_ctx*, const SnapDev&, const SnapRun&, const S*, const double*,     \
                                           const typename BaseOf<S>::type*, double*, double*, S*, S*, double*);
-IPG_SNAP_EXTERN_UD(float)
+extern template void snap_launch_ui<float>(ipg_ctx*, const SnapDev&, const SnapRun&, float*);
 IPG_SNAP_EXTERN(double)
 IPG_SNAP_EXTERN(Dual<float>)
 IPG_SNAP_EXTERN(Dual<double>)

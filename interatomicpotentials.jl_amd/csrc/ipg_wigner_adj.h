@@ -359,8 +359,18 @@ __host__ __device__ __forceinline__ void vjp(const AB<T>& c, Seed& seed, Acc& ac
   });
 }
 
-// Levels JJ+1..J of row MB after level JJ, each emitted in full (values
-// scaled back by N).
+// Position of entry (level j, column ma) among the emitted entries of row MB
+// (columns 0..MB of level 2MB, then every column of levels 2MB+1..J), and
+// the row's entry count.
+constexpr int row_index(int MB, int j, int ma) {
+  int k = MB + 1;
+  if (j == 2 * MB) return ma;
+  for (int l = 2 * MB + 1; l < j; l++) k += l + 1;
+  return k + ma;
+}
+constexpr int row_count(int J, int MB) { return row_index(MB, J + 1, 0); }
+
+// Levels JJ+1..J of row MB after level JJ, each emitted in full.
 template <typename T, int J, int MB, int JJ, typename Emit>
 __host__ __device__ __forceinline__ void fwd_rest(const AB<T>& c, const T (&vr)[JJ + 1], const T (&vi)[JJ + 1],
                                                   Emit& emit) {
@@ -369,38 +379,62 @@ __host__ __device__ __forceinline__ void fwd_rest(const AB<T>& c, const T (&vr)[
     fstep<T, JJ + 1, MB>(c, vr, vi, nr, ni);
     sfor<0, JJ + 2>([&](auto mc) {
       constexpr int ma = decltype(mc)::value;
-      constexpr T nm = (T)norm(JJ + 1, MB, ma);
-      emit(K<JJ + 1>{}, K<MB>{}, mc, nm * nr[ma], nm * ni[ma]);
+      emit(K<JJ + 1>{}, mc, nr[ma], ni[ma]);
     });
-    fwd_rest<T, J, MB, JJ + 1>(c, nr, ni, emit);
+    const AB<T> cn = emit.next(c);
+    fwd_rest<T, J, MB, JJ + 1>(cn, nr, ni, emit);
   }
 }
 
+// The forward recursion of row MB alone, from level 0 (the staircase of the
+// rows below, then the row itself), emitting its entries in row order with
+// their rescaled values v = u / N: emit(K<j>, K<ma>, v_r, v_i).  After each
+// level the chain continues with emit.next(c), through which a caller can
+// make the next level wait for its use of this one.  The
+// staircase up to level 2MB-1 forms MB (2MB+1) entries, the same count as
+// levels 0..2MB-1 of a row, so every row forms (J+1)(J+2)/2 entries (45 at
+// 2J = 8).
+template <typename T, int J, int MB, typename Emit>
+__host__ __device__ __forceinline__ void row_forward(const AB<T>& c, Emit& emit) {
+  if constexpr (MB == 0) {
+    const T vr[1] = {(T)1}, vi[1] = {(T)0};
+    emit(K<0>{}, K<0>{}, vr[0], vi[0]);
+    const AB<T> cn = emit.next(c);
+    fwd_rest<T, J, 0, 0>(cn, vr, vi, emit);
+  } else {
+    T sr[2 * MB], si[2 * MB], tr[2 * MB], ti[2 * MB];
+    stair<T, MB>(c, sr, si);
+    mirror<T, MB>(sr, si, tr, ti);
+    T wr[2 * MB + 1], wi[2 * MB + 1];
+    fstep<T, 2 * MB, MB>(c, tr, ti, wr, wi);
+    middle<T, MB>(wr, wi);
+    sfor<0, MB + 1>([&](auto mc) { emit(K<2 * MB>{}, mc, wr[decltype(mc)::value], wi[decltype(mc)::value]); });
+    const AB<T> cn = emit.next(c);
+    fwd_rest<T, J, MB, 2 * MB>(cn, wr, wi, emit);
+  }
+}
+
+// Adapter for forward(): scales row MB's entries back by N.
+template <typename T, int MB, typename Emit>
+struct ScaledEmit {
+  Emit& emit;
+  template <int j, int ma>
+  __host__ __device__ __forceinline__ void operator()(K<j> jc, K<ma> mc, T vr, T vi) {
+    constexpr T nm = (T)norm(j, MB, ma);
+    emit(jc, K<MB>{}, mc, nm * vr, nm * vi);
+  }
+  __host__ __device__ __forceinline__ AB<T> next(const AB<T>& c) const { return c; }
+};
+
 // The plain forward recursion, emitting every left-half entry with its
-// values (scaled back by N): emit(K<j>, K<mb>, K<ma>, u_r, u_i).  Used by the host-side checks
-// of vjp (Phi by finite differences).
+// values (scaled back by N): emit(K<j>, K<mb>, K<ma>, u_r, u_i).  Used by
+// the host-side checks of vjp.
 template <typename T, int J, typename Emit>
 __host__ __device__ __forceinline__ void forward(const AB<T>& c, Emit& emit) {
   sfor<0, J / 2 + 1>([&](auto mbc) {
     constexpr int MB = decltype(mbc)::value;
-    if constexpr (MB == 0) {
-      const T vr[1] = {(T)1}, vi[1] = {(T)0};
-      emit(K<0>{}, K<0>{}, K<0>{}, vr[0], vi[0]);  // N(0, 0, 0) = 1
-      fwd_rest<T, J, 0, 0>(c, vr, vi, emit);
-    } else {
-      T sr[2 * MB], si[2 * MB], tr[2 * MB], ti[2 * MB];
-      stair<T, MB>(c, sr, si);
-      mirror<T, MB>(sr, si, tr, ti);
-      T wr[2 * MB + 1], wi[2 * MB + 1];
-      fstep<T, 2 * MB, MB>(c, tr, ti, wr, wi);
-      middle<T, MB>(wr, wi);
-      sfor<0, MB + 1>([&](auto mc) {
-        constexpr int ma = decltype(mc)::value;
-        constexpr T nm = (T)norm(2 * MB, MB, ma);
-        emit(K<2 * MB>{}, K<MB>{}, mc, nm * wr[ma], nm * wi[ma]);
-      });
-      fwd_rest<T, J, MB, 2 * MB>(c, wr, wi, emit);
-    }
+    ScaledEmit<T, MB, Emit> e{emit};
+    row_forward<T, J, MB>(c, e);
   });
 }
 
