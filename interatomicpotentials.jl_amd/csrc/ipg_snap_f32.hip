@@ -248,11 +248,13 @@ void snap_launch_ui_rows32(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, flo
 // with grad_r Phi = Re(a_bar) da_r + Im(a_bar) da_i + Re(b_bar) db_r +
 // Im(b_bar) db_i from the FP64 Jacobian of (a, b).  One thread per pair.
 // ---------------------------------------------------------------------------
-// Seeds of the sweep: Y_i,e in the atom-blocked layout.  order() launders a
-// zero offset through an empty asm that depends on x, so a level's reads
-// cannot be hoisted above the point where the sweep reaches it; tie() does
-// the same for a value, so a row's recursion starts after the previous row.
+// Seeds of the sweep: Y_i,e in the atom-blocked layout, already weighted by
+// the Z/Y kernel.  order() launders a zero offset through an empty asm that
+// depends on x, so a level's reads cannot be hoisted above the point where
+// the sweep reaches it; tie() does the same for a value, so a row's
+// recursion starts after the previous row.
 struct YSeed {
+  static constexpr bool weighted = true;
   const float2* y;
   __device__ __forceinline__ int order(float x) const {
     int z = 0;

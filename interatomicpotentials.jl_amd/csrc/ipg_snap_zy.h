@@ -17,9 +17,13 @@
 // FP64 registers.  Z is never stored in the adjoint mode.
 //
 // The U and Y buffers use the atom-blocked layout [batch][jju][32][re, im]
-// written by the FP32 U kernel and read by the FP32 force kernel.
+// written by the FP32 U kernel and read by the FP32 force kernel.  Y holds
+// the entries the force kernel's adjoint sweep seeds (mb < j/2, and
+// ma <= mb on the middle row of an even level), each already multiplied by
+// its seed weight 2 w(e) N(e) (ipg_wigner_adj.h).
 #pragma once
 #include "ipg_snap_kernels.h"
+#include "ipg_wigner_adj.h"
 
 namespace ipg {
 namespace zy {
@@ -209,10 +213,14 @@ __device__ __forceinline__ void triple(const float2* __restrict__ UL, const floa
   constexpr int mb2hi = m2max(j1, j2, j, mb);
   constexpr int nb = mcount(j1, j2, j, mb);
   constexpr BlockPos bp = block_pos(J, j1, j2, j);
+  // columns formed: all of them for the stored Z (DESC); for Y and the
+  // energy the middle row of an even level needs only its left half and
+  // centre (the adjoint sweep and the B half-sums use no other entry)
+  constexpr int NA = (!DESC && 2 * mb == j) ? mb + 1 : j + 1;
   // Z of this half's mb1 rows: the FP32 column sums times the row factor,
   // summed over at most five rows in FP32 and widened to FP64 once
   float zrf[j + 1], zif[j + 1];
-  static_for<0, j + 1>([&](auto mc) {
+  static_for<0, NA>([&](auto mc) {
     zrf[decltype(mc)::value] = 0.f;
     zif[decltype(mc)::value] = 0.f;
   });
@@ -224,7 +232,7 @@ __device__ __forceinline__ void triple(const float2* __restrict__ UL, const floa
     static_for<0, j1 + 1>([&](auto k) { u1[decltype(k)::value] = r1[decltype(k)::value * AB]; });
     static_for<0, j2 + 1>([&](auto k) { u2[decltype(k)::value] = r2[decltype(k)::value * AB]; });
     const float cb = cgL[bp.cgoff + mb1 * (j2 + 1) + mb2];
-    static_for<0, j + 1>([&](auto mc) {
+    static_for<0, NA>([&](auto mc) {
       constexpr int ma = decltype(mc)::value;
       constexpr int lo = m1min(j1, j2, j, ma), hi2 = m2max(j1, j2, j, ma), na = mcount(j1, j2, j, ma);
       float sr = 0.f, si = 0.f;
@@ -274,7 +282,7 @@ __device__ __forceinline__ void triple(const float2* __restrict__ UL, const floa
     }
     return;
   }
-  static_for<0, j + 1>([&](auto mc) {
+  static_for<0, NA>([&](auto mc) {
     zr[decltype(mc)::value] = (double)zrf[decltype(mc)::value];
     zi[decltype(mc)::value] = (double)zif[decltype(mc)::value];
   });
@@ -297,7 +305,7 @@ __device__ __forceinline__ void triple(const float2* __restrict__ UL, const floa
     e += bts[bp.jjb] * ev;
   }
   const double cf = ycs[bp.blk];
-  static_for<0, j + 1>([&](auto mc) {
+  static_for<0, NA>([&](auto mc) {
     yr[decltype(mc)::value] += cf * zr[decltype(mc)::value];
     yi[decltype(mc)::value] += cf * zi[decltype(mc)::value];
     // pin the update here: otherwise it is sunk to the unit's end and the
@@ -312,8 +320,9 @@ __device__ __forceinline__ void unit(const float2* __restrict__ UL, const float*
                                      const double* __restrict__ ycs, const double* __restrict__ bts,
                                      const SnapDev& P, double& e, float2* __restrict__ Yrow, bool store,
                                      float2* __restrict__ Zat, double* __restrict__ Bp) {
+  constexpr int NA = (!DESC && 2 * mb == j) ? mb + 1 : j + 1;  // columns used (see triple)
   double yr[j + 1], yi[j + 1];
-  static_for<0, j + 1>([&](auto mc) {
+  static_for<0, NA>([&](auto mc) {
     yr[decltype(mc)::value] = 0.0;
     yi[decltype(mc)::value] = 0.0;
   });
@@ -327,20 +336,25 @@ __device__ __forceinline__ void unit(const float2* __restrict__ UL, const float*
   });
   if constexpr (DESC) return;
   // the two halves hold partial sums over alternate mb1 rows
-  static_for<0, j + 1>([&](auto mc) {
+  static_for<0, NA>([&](auto mc) {
     constexpr int ma = decltype(mc)::value;
     yr[ma] = half_sum(yr[ma]);
     yi[ma] = half_sum(yi[ma]);
   });
   if (!store) return;
-  // half h stores the columns ma = 2k + h (one instruction per column pair)
-  static_for<0, (j + 2) / 2>([&](auto kc) {
+  // half h stores the columns ma = 2k + h (one instruction per column pair),
+  // each weighted by 2 w(e) N(e), the seed weight of the rescaled adjoint
+  // sweep of the force kernel (its only reader)
+  static_for<0, (NA + 1) / 2>([&](auto kc) {
     constexpr int m0 = 2 * decltype(kc)::value;
-    if constexpr (m0 + 1 <= j) {
-      const float2 v = h ? make_float2((float)yr[m0 + 1], (float)yi[m0 + 1]) : make_float2((float)yr[m0], (float)yi[m0]);
+    constexpr double w0 = wadj::seed_weight(j, mb, m0);
+    if constexpr (m0 + 1 < NA) {
+      constexpr double w1 = wadj::seed_weight(j, mb, m0 + 1);
+      const float2 v = h ? make_float2((float)(w1 * yr[m0 + 1]), (float)(w1 * yi[m0 + 1]))
+                         : make_float2((float)(w0 * yr[m0]), (float)(w0 * yi[m0]));
       Yrow[(m0 + h) * AB] = v;
     } else {
-      if (h == 0) Yrow[m0 * AB] = make_float2((float)yr[m0], (float)yi[m0]);
+      if (h == 0) Yrow[m0 * AB] = make_float2((float)(w0 * yr[m0]), (float)(w0 * yi[m0]));
     }
   });
 }

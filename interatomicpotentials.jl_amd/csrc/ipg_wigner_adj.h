@@ -82,6 +82,11 @@ constexpr double cfact(int n) {
 constexpr double norm(int j, int mb, int ma) {
   return csqrt(cfact(ma) * cfact(j - ma) / (cfact(mb) * cfact(j - mb)));
 }
+// Weight 2 w(e) N(e) of the seed of left-half entry e = (j, mb, ma) in the
+// rescaled sweep (w = 1/2 on the centre of an even level).
+constexpr double seed_weight(int j, int mb, int ma) {
+  return ((j == 2 * mb && ma == mb) ? 1.0 : 2.0) * norm(j, mb, ma);
+}
 
 // Cayley-Klein parameters of one neighbor.
 template <typename T>
@@ -270,8 +275,8 @@ __host__ __device__ __forceinline__ void level(const AB<T>& c, const T (&vr)[JJ 
   sfor<0, JJ + 1>([&](auto mc) {
     constexpr int ma = decltype(mc)::value;
     if constexpr (JJ > 2 * MB || ma <= MB) {
-      // 2 w(e) N(e): w = 1/2 on the centre of an even level
-      constexpr T wn = (T)(((JJ == 2 * MB && ma == MB) ? 1.0 : 2.0) * norm(JJ, MB, ma));
+      // 2 w(e) N(e), unless the seed comes with it (Seed::weighted)
+      constexpr T wn = Seed::weighted ? (T)1 : (T)seed_weight(JJ, MB, ma);
       const T gr = wn * sr[ma], gi = wn * si[ma];
       lr[ma] = lr[ma] + gr;
       li[ma] = li[ma] + gi;
@@ -338,7 +343,8 @@ __host__ __device__ __forceinline__ void row(const AB<T>& c, T (&incr)[J + 1], T
 // (j, mb, ma) as an object with members x (real part) and y (imaginary
 // part), read under the ordering token tok; seed.order(x) makes a token that
 // depends on x (0 where the order does not matter) and seed.tie(v, x)
-// returns v made to depend on x.
+// returns v made to depend on x.  Seed::weighted (constexpr bool) says that
+// the seeds already carry their weights seed_weight(j, mb, ma).
 template <typename T, int J, typename Seed>
 __host__ __device__ __forceinline__ void vjp(const AB<T>& c, Seed& seed, Acc& acc) {
   T incr[J + 1], inci[J + 1];

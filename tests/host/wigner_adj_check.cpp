@@ -62,6 +62,7 @@ static Geo geometry(double x, double y, double z, double rcut, double rfac0, dou
 // Seeds s_e read from the case's idxu-ordered list.
 template <typename T>
 struct Seed {
+  static constexpr bool weighted = false;
   struct S2 {
     T x, y;
   };
