@@ -359,6 +359,294 @@ __device__ __forceinline__ void unit(const float2* __restrict__ UL, const float*
   });
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint schedule with shared products.  The rows (j, mb) with the same
+// d = j - 2mb read the same pairs of U rows in every block (j1, j2):
+// mb1 + mb2 = S = (j1 + j2 - d) / 2, and for a column sum c = ma1 + ma2 row
+// j forms column ma = c - (j1 + j2 - j) / 2 from the same products
+// U_j1(mb1, ma1) U_j2(mb2, ma2), the larger rows' columns covering the
+// smaller rows'.  A piece is a run of rows of one d (a table for 2J = 8,
+// otherwise runs of at most PIECE_CAP columns); the wave that owns it forms
+// each product once and adds it,
+// with each row's literal Clebsch-Gordan factor, into the rows that use it.
+// Every Y row belongs to one piece, so no sums cross waves.
+// ---------------------------------------------------------------------------
+constexpr int PIECE_CAP = 9;
+constexpr int PIECE_ROWS = 5;
+constexpr int row_na(int j, int mb) { return 2 * mb == j ? mb + 1 : j + 1; }  // columns of Y used
+constexpr int cnum(int j1, int j2, int c) { return cmin(j1, c) - cmax(0, c - j2) + 1; }  // products of sum c
+
+// Pieces of 2J = 8: runs of rows (their levels j, largest first) of one
+// d = j - 2mb, chosen offline to minimise the largest wave load after the
+// LPT assignment below (cost as in piece_cost) with at most 16 columns per
+// piece; -1 pads a run.
+constexpr int PIECES_J8_N = 15;
+constexpr int PIECES_J8_D[PIECES_J8_N] = {0, 0, 1, 1, 2, 2, 2, 3, 3, 4, 4, 5, 6, 7, 8};
+constexpr int PIECES_J8[PIECES_J8_N][3] = {{8, 6, 4}, {2, 0, -1},           // d = 0
+                                           {7, 5, -1}, {3, 1, -1},          // d = 1
+                                           {8, -1, -1}, {6, -1, -1}, {4, 2, -1},  // d = 2
+                                           {7, 5, -1}, {3, -1, -1},         // d = 3
+                                           {8, -1, -1}, {6, 4, -1},         // d = 4
+                                           {7, 5, -1},                      // d = 5
+                                           {8, 6, -1},                      // d = 6
+                                           {7, -1, -1},                     // d = 7
+                                           {8, -1, -1}};                    // d = 8
+
+template <int J>
+struct PieceSet {
+  static constexpr int NPMAX = 48;
+  int np = 0;
+  int pd[NPMAX] = {}, pn[NPMAX] = {}, pj[NPMAX][PIECE_ROWS] = {}, cost[NPMAX] = {};
+  constexpr PieceSet() {
+    if (J == 8) {
+      np = PIECES_J8_N;
+      for (int p = 0; p < np; p++) {
+        pd[p] = PIECES_J8_D[p];
+        for (int k = 0; k < 3 && PIECES_J8[p][k] >= 0; k++) pj[p][pn[p]++] = PIECES_J8[p][k];
+        cost[p] = piece_cost(p);
+      }
+      return;
+    }
+    for (int d = 0; d <= J; d++) {
+      int cur = -1, used = 0;
+      for (int j = J - ((J - d) & 1); j >= d; j -= 2) {  // rows of d, largest first
+        const int e = row_na(j, (j - d) / 2);
+        if (cur < 0 || used + e > PIECE_CAP || pn[cur] == PIECE_ROWS) {
+          cur = np++;
+          pd[cur] = d;
+          used = 0;
+        }
+        pj[cur][pn[cur]++] = j;
+        used += e;
+      }
+    }
+    for (int p = 0; p < np; p++) cost[p] = piece_cost(p);
+  }
+  // products, weighted adds and row updates over every block, plus the row
+  // loads and the Y/energy work per row
+  constexpr int piece_cost(int p) const {
+    const int d = pd[p];
+    int c = 0;
+    for (int j1 = 0; j1 <= J; j1++)
+      for (int j2 = 0; j2 <= j1; j2++) {
+        if ((j1 + j2 + d) & 1) continue;
+        const int S = (j1 + j2 - d) / 2;
+        const int nb = cmin(j1, S) - cmax(0, S - j2) + 1;
+        if (nb <= 0) continue;
+        int prod = 0, uses = 0, rows = 0, top = -1;
+        for (int k = 0; k < pn[p]; k++) {
+          const int j = pj[p][k];
+          if (!valid(J, j1, j2, j)) continue;
+          const int off = (j1 + j2 - j) / 2, na = row_na(j, (j - d) / 2);
+          for (int c2 = off; c2 < off + na; c2++) uses += cnum(j1, j2, c2);
+          rows += na;
+          if (top < 0) {
+            top = j;
+            for (int c2 = off; c2 < off + na; c2++) prod += cnum(j1, j2, c2);
+          }
+        }
+        if (top < 0) continue;
+        c += nb * (4 * prod + 2 * uses + 2 * rows + j1 + j2 + 2) + 8 * rows;
+      }
+    return c;
+  }
+};
+
+template <int J, int NW>
+struct PiecePlan {
+  PieceSet<J> ps{};
+  int owner[PieceSet<J>::NPMAX] = {};
+  constexpr PiecePlan() {
+    int order[PieceSet<J>::NPMAX] = {};
+    for (int p = 0; p < ps.np; p++) order[p] = p;
+    for (int a = 0; a < ps.np; a++)  // by decreasing cost
+      for (int b = a + 1; b < ps.np; b++)
+        if (ps.cost[order[b]] > ps.cost[order[a]]) {
+          const int t = order[a];
+          order[a] = order[b];
+          order[b] = t;
+        }
+    int load[NW] = {};
+    for (int k = 0; k < ps.np; k++) {
+      int best = 0;
+      for (int w = 1; w < NW; w++)
+        if (load[w] < load[best]) best = w;
+      owner[order[k]] = best;
+      load[best] += ps.cost[order[k]];
+    }
+  }
+};
+template <int J, int NW>
+struct PiecePlanOf {
+  static constexpr PiecePlan<J, NW> v{};
+};
+
+// Block (j1, j2) of piece P: the Z rows of its rows valid in the block,
+// over this half's row pairs, then each row's energy share and Y update.
+template <int J, int NW, int P, int j1, int j2>
+__device__ __forceinline__ void piece_block(const float2* __restrict__ UL, const float* __restrict__ cgL, int h,
+                                            int at, const double* __restrict__ ycs, const double* __restrict__ bts,
+                                            const SnapDev& Pd, double (&yr)[PIECE_ROWS][J + 1],
+                                            double (&yi)[PIECE_ROWS][J + 1], double& e) {
+  constexpr auto& ps = PiecePlanOf<J, NW>::v.ps;
+  constexpr int d = ps.pd[P], NR = ps.pn[P];
+  constexpr int S = (j1 + j2 - d) / 2;
+  constexpr int lo = cmax(0, S - j2), nb = cmin(j1, S) - lo + 1;
+  float zr[PIECE_ROWS][J + 1], zi[PIECE_ROWS][J + 1];
+  static_for<0, NR>([&](auto kc) {
+    constexpr int k = decltype(kc)::value, j = ps.pj[P][k];
+    if constexpr (valid(J, j1, j2, j))
+      static_for<0, row_na(j, (j - d) / 2)>([&](auto mc) {
+        zr[k][decltype(mc)::value] = 0.f;
+        zi[k][decltype(mc)::value] = 0.f;
+      });
+  });
+  // the largest row valid in the block spans every column sum the others use
+  constexpr int top = [] {
+    for (int k = 0; k < NR; k++)
+      if (valid(J, j1, j2, ps.pj[P][k])) return k;
+    return -1;
+  }();
+  static_assert(top >= 0, "piece without a row in the block");
+  constexpr int jt = ps.pj[P][top];
+  constexpr int c0 = (j1 + j2 - jt) / 2, c1 = c0 + row_na(jt, (jt - d) / 2);
+  for (int ib = h; ib < nb; ib += 2) {
+    const int mb1 = lo + ib, mb2 = S - mb1;
+    const float2* r1 = UL + (ub_of(j1) + mb1 * (j1 + 1)) * AB + at;
+    const float2* r2 = UL + (ub_of(j2) + mb2 * (j2 + 1)) * AB + at;
+    float2 u1[j1 + 1], u2[j2 + 1];
+    static_for<0, j1 + 1>([&](auto k) { u1[decltype(k)::value] = r1[decltype(k)::value * AB]; });
+    static_for<0, j2 + 1>([&](auto k) { u2[decltype(k)::value] = r2[decltype(k)::value * AB]; });
+    float cb[PIECE_ROWS];
+    static_for<0, NR>([&](auto kc) {
+      constexpr int k = decltype(kc)::value, j = ps.pj[P][k];
+      if constexpr (valid(J, j1, j2, j)) cb[k] = cgL[block_pos(J, j1, j2, j).cgoff + mb1 * (j2 + 1) + mb2];
+    });
+    static_for<c0, c1>([&](auto cc) {
+      constexpr int c = decltype(cc)::value;
+      float sr[PIECE_ROWS], si[PIECE_ROWS];
+      static_for<0, NR>([&](auto kc) {
+        sr[decltype(kc)::value] = 0.f;
+        si[decltype(kc)::value] = 0.f;
+      });
+      static_for<cmax(0, c - j2), cmin(j1, c) + 1>([&](auto ac) {
+        constexpr int ma1 = decltype(ac)::value, ma2 = c - ma1;
+        const float2 p = u1[ma1], q = u2[ma2];
+        const float pr = p.x * q.x - p.y * q.y, pi = p.x * q.y + p.y * q.x;
+        static_for<0, NR>([&](auto kc) {
+          constexpr int k = decltype(kc)::value, j = ps.pj[P][k];
+          if constexpr (valid(J, j1, j2, j)) {
+            constexpr int ma = c - (j1 + j2 - j) / 2;
+            if constexpr (ma >= 0 && ma < row_na(j, (j - d) / 2)) {
+              constexpr float w = (float)cgm(j1, j2, j, ma1, ma2);
+              sr[k] += w * pr;
+              si[k] += w * pi;
+            }
+          }
+        });
+      });
+      static_for<0, NR>([&](auto kc) {
+        constexpr int k = decltype(kc)::value, j = ps.pj[P][k];
+        if constexpr (valid(J, j1, j2, j)) {
+          constexpr int ma = c - (j1 + j2 - j) / 2;
+          if constexpr (ma >= 0 && ma < row_na(j, (j - d) / 2)) {
+            zr[k][ma] += cb[k] * sr[k];
+            zi[k][ma] += cb[k] * si[k];
+          }
+        }
+      });
+    });
+  }
+  static_for<0, NR>([&](auto kc) {
+    constexpr int k = decltype(kc)::value, j = ps.pj[P][k], mb = (j - d) / 2;
+    if constexpr (valid(J, j1, j2, j)) {
+      constexpr int NA = row_na(j, mb);
+      constexpr BlockPos bp = block_pos(J, j1, j2, j);
+      double z64r[NA], z64i[NA];
+      static_for<0, NA>([&](auto mc) {
+        z64r[decltype(mc)::value] = (double)zr[k][decltype(mc)::value];
+        z64i[decltype(mc)::value] = (double)zi[k][decltype(mc)::value];
+      });
+      if constexpr (j >= j1) {
+        // this half's share of B_(j1,j2,j) = 2 sum_half Re(conj(U_j) Z)
+        const float2* uj = UL + (ub_of(j) + mb * (j + 1)) * AB + at;
+        double bs = 0.0;
+        static_for<0, NA>([&](auto mc) {
+          constexpr int ma = decltype(mc)::value;
+          constexpr double wgt = (2 * mb < j) ? 1.0 : (ma < mb ? 1.0 : 0.5);
+          const float2 uu = uj[ma * AB];
+          bs += wgt * ((double)uu.x * z64r[ma] + (double)uu.y * z64i[ma]);
+        });
+        double ev = 2.0 * bs;
+        if constexpr (mb == 0)
+          if (h == 0 && Pd.bzero) ev -= Pd.bzero_j[j];
+        e += bts[bp.jjb] * ev;
+      }
+      const double cf = ycs[bp.blk];
+      static_for<0, NA>([&](auto mc) {
+        constexpr int ma = decltype(mc)::value;
+        yr[k][ma] += cf * z64r[ma];
+        yi[k][ma] += cf * z64i[ma];
+        asm volatile("" : "+v"(yr[k][ma]), "+v"(yi[k][ma]));
+      });
+      asm volatile("" : "+v"(e));
+    }
+  });
+}
+
+// Piece P: every block, then the rows' Y (both halves' sums) stored with
+// the force kernel's seed weights.
+template <int J, int NW, int P>
+__device__ __forceinline__ void piece(const float2* __restrict__ UL, const float* __restrict__ cgL, int h, int at,
+                                      const double* __restrict__ ycs, const double* __restrict__ bts,
+                                      const SnapDev& Pd, double& e, float2* __restrict__ yb, bool store) {
+  constexpr auto& ps = PiecePlanOf<J, NW>::v.ps;
+  constexpr int d = ps.pd[P], NR = ps.pn[P];
+  double yr[PIECE_ROWS][J + 1], yi[PIECE_ROWS][J + 1];
+  static_for<0, NR>([&](auto kc) {
+    constexpr int k = decltype(kc)::value, j = ps.pj[P][k];
+    static_for<0, row_na(j, (j - d) / 2)>([&](auto mc) {
+      yr[k][decltype(mc)::value] = 0.0;
+      yi[k][decltype(mc)::value] = 0.0;
+    });
+  });
+  static_for<0, J + 1>([&](auto c1) {
+    constexpr int j1 = decltype(c1)::value;
+    static_for<0, j1 + 1>([&](auto c2) {
+      constexpr int j2 = decltype(c2)::value;
+      constexpr bool any = [] {
+        for (int k = 0; k < NR; k++)
+          if (valid(J, j1, j2, ps.pj[P][k])) return true;
+        return false;
+      }();
+      if constexpr (any) piece_block<J, NW, P, j1, j2>(UL, cgL, h, at, ycs, bts, Pd, yr, yi, e);
+    });
+  });
+  static_for<0, NR>([&](auto kc) {
+    constexpr int k = decltype(kc)::value, j = ps.pj[P][k], mb = (j - d) / 2, NA = row_na(j, mb);
+    static_for<0, NA>([&](auto mc) {
+      constexpr int ma = decltype(mc)::value;
+      yr[k][ma] = half_sum(yr[k][ma]);
+      yi[k][ma] = half_sum(yi[k][ma]);
+    });
+    if (!store) return;
+    float2* Yrow = yb + (ub_of(j) + mb * (j + 1)) * AB;
+    // half h stores the columns ma = 2k + h, weighted as in unit()
+    static_for<0, (NA + 1) / 2>([&](auto tc) {
+      constexpr int m0 = 2 * decltype(tc)::value;
+      constexpr double w0 = wadj::seed_weight(j, mb, m0);
+      if constexpr (m0 + 1 < NA) {
+        constexpr double w1 = wadj::seed_weight(j, mb, m0 + 1);
+        const float2 v = h ? make_float2((float)(w1 * yr[k][m0 + 1]), (float)(w1 * yi[k][m0 + 1]))
+                           : make_float2((float)(w0 * yr[k][m0]), (float)(w0 * yi[k][m0]));
+        Yrow[(m0 + h) * AB] = v;
+      } else {
+        if (h == 0) Yrow[m0 * AB] = make_float2((float)(w0 * yr[k][m0]), (float)(w0 * yi[k][m0]));
+      }
+    });
+  });
+}
+
 }  // namespace zy
 
 // DESC = false: energy per atom and Y (adjoint).  DESC = true: Z in the
@@ -407,13 +695,18 @@ __global__ __launch_bounds__(64 * NW) void k_snap_zy32(SnapDev P, int64_t n, con
   float2* yb = DESC ? nullptr : Yb + batch * (int64_t)IDXU * zy::AB + at;
   float2* zat = DESC ? Zout + (valid ? a : 0) * (int64_t)bsl.nz : nullptr;
   double e = 0.0;
-  static_for<0, zy::Plan<J, NW>::NU>([&](auto uc) {
-    constexpr int u = decltype(uc)::value;
-    constexpr int j = plan.uj[u], mb = plan.umb[u];
-    if (w == plan.owner[u])
-      zy::unit<J, j, mb, DESC>(UL, cgL, h, at, ycs, bts, P, e,
-                               DESC ? nullptr : yb + (ub_of(j) + mb * (j + 1)) * zy::AB, valid, zat, BpL + at);
-  });
+  if constexpr (DESC) {
+    static_for<0, zy::Plan<J, NW>::NU>([&](auto uc) {
+      constexpr int u = decltype(uc)::value;
+      constexpr int j = plan.uj[u], mb = plan.umb[u];
+      if (w == plan.owner[u]) zy::unit<J, j, mb, DESC>(UL, cgL, h, at, ycs, bts, P, e, nullptr, valid, zat, BpL + at);
+    });
+  } else {
+    static_for<0, zy::PiecePlanOf<J, NW>::v.ps.np>([&](auto pc) {
+      constexpr int p = decltype(pc)::value;
+      if (w == zy::PiecePlanOf<J, NW>::v.owner[p]) zy::piece<J, NW, p>(UL, cgL, h, at, ycs, bts, P, e, yb, valid);
+    });
+  }
   if constexpr (DESC) {
     __syncthreads();
     // B_jjb = sum over its row slots, in row order, minus bzero
