@@ -111,28 +111,19 @@ __device__ __forceinline__ void ck64(double x, double y, double z, double rcut, 
   }
 }
 
-// Emitter of one row chain: adds w v_e to the row's sums.  Each level's
-// sums are pinned where they are formed and the chain's next level waits
-// for them (next), so only one or two levels of the chain are live at once.
+// Emitter of one row chain: adds w v_e to the row's sums.
 template <int MB>
 struct RowSum {
   float* accr;
   float* acci;
   float w;
-  int last;
   template <int j, int ma>
   __device__ __forceinline__ void operator()(IC<j>, IC<ma>, float vr, float vi) {
     constexpr int k = wadj::row_index(MB, j, ma);
     accr[k] = accr[k] + w * vr;
     acci[k] = acci[k] + w * vi;
-    asm volatile("" : "+v"(accr[k]), "+v"(acci[k]));
-    last = k;
   }
-  __device__ __forceinline__ wadj::AB<float> next(const wadj::AB<float>& c) const {
-    wadj::AB<float> t = c;
-    asm volatile("" : "+v"(t.ar), "+v"(t.ai), "+v"(t.br), "+v"(t.bi) : "v"(accr[last]));
-    return t;
-  }
+  __device__ __forceinline__ wadj::AB<float> next(const wadj::AB<float>& c) const { return c; }
 };
 
 template <int J>
@@ -172,7 +163,7 @@ __global__ __launch_bounds__(64 * (J / 2 + 1)) void k_snap_ui_rows32(SnapDev P, 
 #pragma unroll 1
     for (int64_t q = q0 + s; q < q1; q += 4) {
       const float4 ab = ckab[q];
-      RowSum<MB> emit{accr, acci, ckw[q], 0};
+      RowSum<MB> emit{accr, acci, ckw[q]};
       wadj::row_forward<float, J, MB>(wadj::AB<float>{ab.x, ab.y, ab.z, ab.w}, emit);
     }
   });
