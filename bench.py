@@ -266,7 +266,7 @@ def batch_bench(steps, warmup, nconfig, cpu_configs):
     prof_enable(False)
     set_graphs(True)
     res["kernels_fp32_ms"] = {k: v[1] for k, v in kernel_avg(
-        ["batch_scan", "batch_list_finish", "snap_ui_fp32", "snap_zb_fp32", "snap_dbdr_fp32", "snap_db_gather",
+        ["batch_scan", "batch_list_finish", "batch_list_rev", "snap_ui_fp32", "snap_zb_fp32", "snap_dbdr_fp32", "snap_db_gather",
          "batch_erow", "batch_w_atom", "batch_wrow", "scan", "fill_zero"]).items()}
     from ipgpu import perfmodel
 
@@ -576,7 +576,7 @@ def main():
     # FP32 adjoint.  First a per-kernel breakdown: the step launched kernel
     # by kernel (graphs off) with HIP events around every launch.
     step32 = lambda: snap_step(sn, dsys, d, IPG_SNAP_ADJOINT_FP32)
-    names32 = ["snap_ui_fp32", "snap_zy_fp32", "snap_dedr_fp32", "snap_force_gather", "snap_list_finish",
+    names32 = ["snap_ui_fp32", "snap_zy_fp32", "snap_dedr_fp32", "snap_force_gather", "snap_list_finish", "snap_list_rev",
                "snap_pairs", "nlist_count", "nlist_fill", "nlist_sort", "cell_assign", "cell_place", "cell_rank",
                "fill_zero", "reduce_rows", "scan"]
     set_graphs(False)

@@ -921,12 +921,7 @@ __global__ void k_snapc_pairs(int64_t n, const int64_t* __restrict__ off, const 
 // Sorted CUTOFF-mode rows from the first pass's row buffer: each group of LP
 // lanes stages atom i's hits in LDS, ranks them (the sorted order of the
 // inrange list), and writes nbr, center, r = x_j - x_i (get_displacement,
-// utl.jl:84) and optionally |r|.  With rev, also the index of the reverse
-// pair (j -> i) of each pair: off[j] plus the rank of i among j's hits,
-// counted in j's unsorted row of the same buffer (complete once the scan has
-// run), so no second pass over the sorted rows is needed.  A pair whose
-// reverse is missing (r.r within an ulp of the cutoff under wrapping, or j's
-// row cut by a capacity) points at itself and sets bit 4 of *flag.
+// utl.jl:84) and optionally |r|.
 template <int LP>
 __global__ __launch_bounds__(256) void k_snapc_finish(int64_t n, int K, const int32_t* __restrict__ ell,
                                                       const int32_t* __restrict__ cnt,
@@ -934,8 +929,7 @@ __global__ __launch_bounds__(256) void k_snapc_finish(int64_t n, int K, const in
                                                       IpgBox box1, const int32_t* __restrict__ cfg,
                                                       const IpgBox* __restrict__ boxes, int32_t* __restrict__ nbr,
                                                       int32_t* __restrict__ center, double* __restrict__ rvec,
-                                                      double* __restrict__ R, int32_t* __restrict__ rev,
-                                                      int32_t* __restrict__ flag) {
+                                                      double* __restrict__ R) {
 #pragma clang fp contract(off)
   extern __shared__ int32_t rowbuf[];  // [256 / LP][K]
   const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -970,24 +964,31 @@ __global__ __launch_bounds__(256) void k_snapc_finish(int64_t n, int K, const in
     rvec[3 * q + 1] = ry;
     rvec[3 * q + 2] = rz;
     if (R) R[q] = sqrt(rx * rx + ry * ry + rz * rz);
-    if (rev) {
-      const int cj = min(cnt[j], K);
-      const int32_t* rj = ell + (int64_t)j * K;
-      int below = 0;
-      bool found = false;
-      for (int m = 0; m < cj; m++) {
-        const int32_t v = rj[m];
-        below += v < (int32_t)i;
-        found |= v == (int32_t)i;
-      }
-      const int64_t qr = off[j] + below;
-      if (found && qr < off[j + 1]) {
-        rev[q] = (int32_t)qr;
-      } else {
-        rev[q] = (int32_t)q;
-        atomicOr(flag, 4);
-      }
-    }
+  }
+}
+
+// rev[q]: the pair j -> i of pair q = (i -> j), by binary search in j's
+// sorted row (the rows are complete once the finish kernel has run).  A
+// pair whose reverse is missing (r.r within an ulp of the cutoff under
+// wrapping, or j's row cut by a capacity) points at itself and sets bit 4.
+__global__ void k_rev_search(int64_t np_max, int64_t n, const int64_t* __restrict__ off,
+                             const int32_t* __restrict__ nbr, const int32_t* __restrict__ center,
+                             int32_t* __restrict__ rev, int32_t* __restrict__ flag) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= np_max || q >= off[n]) return;
+  const int32_t i = center[q], j = nbr[q];
+  const int64_t end = off[j + 1];
+  int64_t lo = off[j], hi = end;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (nbr[mid] < i) lo = mid + 1;
+    else hi = mid;
+  }
+  if (lo < end && nbr[lo] == i) {
+    rev[q] = (int32_t)lo;
+  } else {
+    rev[q] = (int32_t)q;
+    atomicOr(flag, 4);
   }
 }
 
@@ -1026,13 +1027,16 @@ SnapListDev build_snap_list(ipg_ctx* ctx, const DevSystem& s, const int32_t* d_a
       launch(ctx, "snap_list_finish", [&] {
         if (LP == 64)
           k_snapc_finish<64><<<grid_for(n * 64, 256), 256, sm, ctx->stream>>>(
-              n, K, L.ell, L.cnt, L.off, s.pos, s.box, nullptr, nullptr, L.nbr, out.center, out.rvec, out.R, out.rev,
-              flag);
+              n, K, L.ell, L.cnt, L.off, s.pos, s.box, nullptr, nullptr, L.nbr, out.center, out.rvec, out.R);
         else
           k_snapc_finish<32><<<grid_for(n * 32, 256), 256, sm, ctx->stream>>>(
-              n, K, L.ell, L.cnt, L.off, s.pos, s.box, nullptr, nullptr, L.nbr, out.center, out.rvec, out.R, out.rev,
-              flag);
+              n, K, L.ell, L.cnt, L.off, s.pos, s.box, nullptr, nullptr, L.nbr, out.center, out.rvec, out.R);
       });
+      if (out.rev)
+        launch(ctx, "snap_list_rev", [&] {
+          k_rev_search<<<grid_for(L.npairs, 256), 256, 0, ctx->stream>>>(L.npairs, n, L.off, L.nbr, out.center,
+                                                                        out.rev, flag);
+        });
     } else if (n > 0 && L.npairs > 0) {
       launch(ctx, "snap_pairs", [&] {
         k_snapc_pairs<<<grid_for(L.npairs, 256), 256, 0, ctx->stream>>>(n, L.off, L.nbr, s.pos, s.box, out.center,
@@ -1212,11 +1216,15 @@ SnapListDev build_snap_list_batch(ipg_ctx* ctx, const BatchSys& b, const SnapCut
     launch(ctx, "batch_list_finish", [&] {
       if (FL == 64)
         k_snapc_finish<64><<<grid_for(n * 64, 256), 256, sm, ctx->stream>>>(
-            n, K, ell, cnt, L.off, b.pos, none, b.cfg, b.boxes, L.nbr, out.center, out.rvec, out.R, out.rev, flag);
+            n, K, ell, cnt, L.off, b.pos, none, b.cfg, b.boxes, L.nbr, out.center, out.rvec, out.R);
       else
         k_snapc_finish<32><<<grid_for(n * 32, 256), 256, sm, ctx->stream>>>(
-            n, K, ell, cnt, L.off, b.pos, none, b.cfg, b.boxes, L.nbr, out.center, out.rvec, out.R, out.rev, flag);
+            n, K, ell, cnt, L.off, b.pos, none, b.cfg, b.boxes, L.nbr, out.center, out.rvec, out.R);
     });
+    if (out.rev)
+      launch(ctx, "batch_list_rev", [&] {
+        k_rev_search<<<grid_for(np, 256), 256, 0, ctx->stream>>>(np, n, L.off, L.nbr, out.center, out.rev, flag);
+      });
   }
   if (need & SNAPL_REVERSE) {
     out.roff = scratch_t<int64_t>(ctx, (t + "_roff").c_str(), n + 1);

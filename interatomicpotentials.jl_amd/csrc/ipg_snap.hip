@@ -359,25 +359,20 @@ __global__ __launch_bounds__(256) void k_force_gather16(int64_t n, const int64_t
   }
 }
 
-// The last kernel of the adjoint step over a symmetric CUTOFF list: the
-// forces, the own row of dE_i/dr_ij minus the reverse pairs rev[q], 16 lanes
-// per atom in a fixed tree (as k_force_gather16<true>, bit for bit), and
-// each block's sum of its atoms' E_i in atom order.  Each block then takes a
-// ticket; the block that takes the last one adds the block sums in block
-// order (so the energy does not depend on timing), writes beta0 + sum, copies
-// the step's flag word to host-visible memory (so the host needs no
-// device-to-host copy after the step) and resets the ticket for the next
-// step.
+// The last two kernels of the adjoint step over a symmetric CUTOFF list.
+// k_step_forces: the forces, the own row of dE_i/dr_ij minus the reverse
+// pairs rev[q], 16 lanes per atom in a fixed tree (as k_force_gather16<true>,
+// bit for bit), and each block's sum of its atoms' E_i in atom order.
+// k_step_close (one block): the energy beta0 + the block sums in block
+// order (so the sum does not depend on timing) and the step's flag word
+// copied to host-visible memory, so the host needs no device-to-host copy
+// after it.  The kernel boundary orders the two; no device-wide fence.
 constexpr int EPI_ATOMS = 256 / FG_LANES;
 __global__ __launch_bounds__(256) void k_step_forces(int64_t n, const int64_t* __restrict__ off,
                                                      const int32_t* __restrict__ rev,
                                                      const double* __restrict__ dedr, double* __restrict__ F,
-                                                     const double* __restrict__ eatom, double* part,
-                                                     int32_t* ticket, double beta0, double* __restrict__ energy,
-                                                     const int32_t* flag, int32_t* host_flag) {
+                                                     const double* __restrict__ eatom, double* __restrict__ part) {
   __shared__ double es[EPI_ATOMS];
-  __shared__ double ws[4];
-  __shared__ int last;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t i = t / FG_LANES;
   const int l = (int)(t % FG_LANES);
@@ -412,23 +407,22 @@ __global__ __launch_bounds__(256) void k_step_forces(int64_t n, const int64_t* _
     double s = 0.0;
     for (int k = 0; k < EPI_ATOMS; k++) s += es[k];
     part[blockIdx.x] = s;
-    __threadfence();  // the block sum is visible device-wide before the ticket
-    last = atomicAdd(ticket, 1) == (int)gridDim.x - 1;
   }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  const volatile double* vp = part;  // written by other blocks: read past this CU's cache
+}
+
+__global__ __launch_bounds__(256) void k_step_close(int64_t nblocks, const double* __restrict__ part, double beta0,
+                                                    double* __restrict__ energy, const int32_t* __restrict__ flag,
+                                                    int32_t* host_flag) {
+  __shared__ double ws[4];
   double s = 0.0;
-  for (int64_t b = threadIdx.x; b < gridDim.x; b += blockDim.x) s += vp[b];
+  for (int64_t b = threadIdx.x; b < nblocks; b += blockDim.x) s += part[b];
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
   if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
   __syncthreads();
   if (threadIdx.x == 0) {
     const double tot = ((ws[0] + ws[1]) + ws[2]) + ws[3];
     if (energy) *energy = beta0 + tot;
-    *host_flag = *(const volatile int32_t*)flag;
-    *ticket = 0;
+    *host_flag = *flag;
   }
 }
 
@@ -838,12 +832,14 @@ static bool energy_forces_dev(ipg_snap* sn, const SnapCall& c, const double* d_b
       fail(IPG_ERR_INVALID, "unknown SNAP path");
     }
     if (c.R.rev && !d_virial && sn->host_flag) {
-      // forces, energy and the flag word in one kernel
+      // forces, energy and the flag word in two kernels
       const unsigned gE = grid_for(n * FG_LANES, 256);
       double* part = scratch_t<double>(ctx, "snap_epart", gE);
       launch(ctx, "snap_force_gather", [&] {
-        k_step_forces<<<gE, 256, 0, ctx->stream>>>(n, c.R.off, c.R.rev, dedr, F, eatom, part, sn->ticket, beta0,
-                                                   d_energy, c.flag, sn->host_flag);
+        k_step_forces<<<gE, 256, 0, ctx->stream>>>(n, c.R.off, c.R.rev, dedr, F, eatom, part);
+      });
+      launch(ctx, "snap_step_close", [&] {
+        k_step_close<<<1, 256, 0, ctx->stream>>>(gE, part, beta0, d_energy, c.flag, sn->host_flag);
       });
       return true;
     }
@@ -1076,16 +1072,9 @@ int ipg_snap_create(ipg_ctx* ctx, const ipg_snap_params* params, ipg_snap** out)
         fail(IPG_ERR_NOMEM, "hipHostMalloc of the SNAP status word failed");
       }
       *sn->host_flag = 0;
-      if (hipMalloc((void**)&sn->ticket, 256) != hipSuccess) {
-        (void)hipGetLastError();
-        sn->ticket = nullptr;
-        fail(IPG_ERR_NOMEM, "hipMalloc of the SNAP step ticket failed");
-      }
-      IPG_HIP(hipMemset(sn->ticket, 0, 256));
     } catch (...) {
       if (sn->dmem) (void)hipFree(sn->dmem);
       if (sn->host_flag) (void)hipHostFree(sn->host_flag);
-      if (sn->ticket) (void)hipFree(sn->ticket);
       delete sn;
       throw;
     }
@@ -1103,7 +1092,6 @@ int ipg_snap_destroy(ipg_snap* sn) {
     if (sn->dmem) (void)hipFree(sn->dmem);
     if (sn->beta_dev) (void)hipFree(sn->beta_dev);
     if (sn->host_flag) (void)hipHostFree(sn->host_flag);
-    if (sn->ticket) (void)hipFree(sn->ticket);
     delete sn;
   });
 }

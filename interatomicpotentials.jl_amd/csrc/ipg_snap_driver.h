@@ -79,5 +79,4 @@ struct ipg_snap {
   int bprec = -1;              // precision the captured batch pipeline was built for
   int64_t bpairs = 0;          // SNAP pairs of the last exact batched run
   int32_t* host_flag = nullptr;  // pinned host word: the flag of the last step that ran the epilogue
-  int32_t* ticket = nullptr;     // device counter of the epilogue's blocks (0 between steps)
 };
