@@ -577,8 +577,8 @@ def main():
     # by kernel (graphs off) with HIP events around every launch.
     step32 = lambda: snap_step(sn, dsys, d, IPG_SNAP_ADJOINT_FP32)
     names32 = ["snap_ui_fp32", "snap_zy_fp32", "snap_dedr_fp32", "snap_force_gather", "snap_list_finish", "snap_list_rev",
-               "snap_pairs", "nlist_count", "nlist_fill", "nlist_sort", "cell_assign", "cell_place", "cell_rank",
-               "fill_zero", "reduce_rows", "scan"]
+               "snap_pairs", "nlist_rows", "cell_pack", "nlist_count", "nlist_fill", "nlist_sort", "cell_assign",
+               "cell_place", "cell_rank", "fill_zero", "reduce_rows", "scan"]
     set_graphs(False)
     for _ in range(a.warmup):
         step32()

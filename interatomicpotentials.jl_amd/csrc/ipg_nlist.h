@@ -25,6 +25,7 @@ struct DevList {
   int ellK_req = 0;        // row width the first pass used (0: no row buffer)
   const int32_t* ell = nullptr;
   const int32_t* cnt = nullptr;
+  bool ell_sorted = false; // the rows in ell are sorted by j
   bool deferred = false;   // rows returned before the overflow bit was read: npairs is an estimate, off is
                            // unset, and the caller reads bit 8 of its flag after using the rows
 };

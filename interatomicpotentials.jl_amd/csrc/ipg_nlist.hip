@@ -426,6 +426,178 @@ __global__ __launch_bounds__(256) void k_list_scan(int64_t n, Grid g, Prefilter 
   }
 }
 
+// Cell-pair rows for small periodic SNAP lists (every dimension periodic
+// with 3..1023 cells, so the 27 cells around a cell are distinct).  No sort
+// by cell: k_cell_pack writes each atom's cell, its exact coordinates with
+// the species index in w, and its FP32 wrapped coordinates.  Then one block
+// per home cell reads every atom's cell (thread t a contiguous range, all
+// loads in flight at once), tests it against a bitmap of the 27 cells
+// around the home cell, and compacts the candidates into LDS in atom order
+// (block prefix sum).  Half-waves take the home cell's atoms: the FP32
+// prefilter of k_list_scan over every candidate, then the exact tests
+// (PeriodicEuclidean and the species-pair cutoff) on the survivors, whose
+// order is kept, so each row comes out sorted by j and goes straight to
+// ell.  cnt[i] = row length (clamped to K, with bit 8 of *flag on
+// overflow); a cell holding more than CR_MAXH atoms, a neighborhood of
+// more than CR_MAXC, or more than CR_KMAX prefilter survivors also set
+// bit 8, and the caller falls back to the binned scan.
+constexpr int CR_MAXC = 768, CR_MAXH = 64, CR_KMAX = 128, CR_UMAX = 16, CR_MAXCELL = 16384;
+__global__ void k_cell_pack(int64_t n, const double* __restrict__ pos, const int32_t* __restrict__ species, Grid g,
+                            int32_t* __restrict__ cellc, double4* __restrict__ pw, float4* __restrict__ pf,
+                            int32_t* __restrict__ clear) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (clear && i == 0) *clear = 0;
+  if (i >= n) return;
+  const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+  cellc[i] = (int32_t)(((int64_t)cell_coord(g, 2, z) * g.nc[1] + cell_coord(g, 1, y)) * g.nc[0] + cell_coord(g, 0, x));
+  pw[i] = make_double4(x, y, z, species ? (double)species[i] : 0.0);
+  double w[3] = {x, y, z};
+  for (int d = 0; d < 3; d++) w[d] = w[d] - floor(w[d] / g.box.L[d]) * g.box.L[d];
+  pf[i] = make_float4((float)w[0], (float)w[1], (float)w[2], 0.f);
+}
+
+__global__ __launch_bounds__(256) void k_snapc_cell_rows(int32_t n, Grid g, Prefilter pfl,
+                                                         const int4* __restrict__ cellc4,
+                                                         const double4* __restrict__ pw,
+                                                         const float4* __restrict__ pf, double smax, SnapCutDev sc,
+                                                         int K, int32_t* __restrict__ ell, int32_t* __restrict__ cnt,
+                                                         int32_t* __restrict__ flag) {
+  __shared__ int32_t cid[CR_MAXC];
+  __shared__ float4 cposf[CR_MAXC];
+  __shared__ uint32_t bitmap[CR_MAXCELL / 32];
+  __shared__ int32_t hid[CR_MAXH];  // LDS slots of the home cell's atoms
+  __shared__ int32_t rows[8][CR_KMAX];
+  __shared__ int32_t wtot[4];
+  __shared__ int32_t nhome;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int h = blockIdx.x;
+  const int nwords = (int)((g.ncell + 31) >> 5);
+  for (int w = t; w < nwords; w += 256) bitmap[w] = 0u;
+  if (t == 0) nhome = 0;
+  __syncthreads();
+  if (t < 27) {
+    const int hx = h % g.nc[0], hy = (h / g.nc[0]) % g.nc[1], hz = h / (g.nc[0] * g.nc[1]);
+    int x = hx + t % 3 - 1, y = hy + (t / 3) % 3 - 1, z = hz + t / 9 - 1;
+    x += x < 0 ? g.nc[0] : (x >= g.nc[0] ? -g.nc[0] : 0);
+    y += y < 0 ? g.nc[1] : (y >= g.nc[1] ? -g.nc[1] : 0);
+    z += z < 0 ? g.nc[2] : (z >= g.nc[2] ? -g.nc[2] : 0);
+    const int c = (z * g.nc[1] + y) * g.nc[0] + x;
+    atomicOr(&bitmap[c >> 5], 1u << (c & 31));
+  }
+  // thread t reads the cells of atoms [4 per t, 4 (per t + per)), per int4
+  // loads of 4 atoms; bit 4u+e of the masks is atom 4 (per t + u) + e
+  const int nq = (n + 3) >> 2;
+  const int per = (nq + 255) >> 8;
+  int4 v[CR_UMAX];
+#pragma unroll
+  for (int u = 0; u < CR_UMAX; u++)
+    if (u < per && per * t + u < nq) v[u] = cellc4[per * t + u];
+  __syncthreads();  // bitmap complete
+  uint64_t cm = 0, hm = 0;
+#pragma unroll
+  for (int u = 0; u < CR_UMAX; u++) {
+    if (u < per && per * t + u < nq) {
+      const int vv[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const bool in = 4 * (per * t + u) + e < n;
+        const int c = in ? vv[e] : 0;  // the padding after atom n - 1 is not written
+        const uint64_t bit = 1ull << (4 * u + e);
+        if (in && ((bitmap[c >> 5] >> (c & 31)) & 1u)) cm |= bit;
+        if (in && c == h) hm |= bit;
+      }
+    }
+  }
+  // block exclusive prefix of the candidate counts (thread order = atom order)
+  const int mine = __popcll(cm);
+  int incl = mine;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += y;
+  }
+  if (lane == 63) wtot[wv] = incl;
+  __syncthreads();
+  int base = incl - mine;
+  for (int w = 0; w < wv; w++) base += wtot[w];
+  const int nc = wtot[0] + wtot[1] + wtot[2] + wtot[3];
+  if (nc > CR_MAXC) {
+    if (t == 0) atomicOr(flag, 8);
+    return;  // uniform: nc is the block total
+  }
+  while (cm) {
+    const int b = __builtin_ctzll(cm);
+    const int32_t k = 4 * (per * t + (b >> 2)) + (b & 3);
+    cid[base] = k;
+    cposf[base] = pf[k];
+    if ((hm >> b) & 1) {
+      const int sl = atomicAdd(&nhome, 1);
+      if (sl < CR_MAXH) hid[sl] = base;
+    }
+    base++;
+    cm &= cm - 1;
+  }
+  __syncthreads();
+  const int nh = nhome;
+  if (nh > CR_MAXH) {
+    if (t == 0) atomicOr(flag, 8);
+    return;
+  }
+  // half-wave hw (0, 1) of wave wv takes home atoms 2 (wv + 4 it) + hw
+  const int hw = lane >> 5, hl = lane & 31;
+  const uint32_t hbelow = (1u << hl) - 1u;
+  int32_t* row = rows[2 * wv + hw];
+  for (int p = wv; 2 * p < nh; p += 4) {
+    const bool act = 2 * p + hw < nh;
+    const int si = act ? hid[2 * p + hw] : 0;
+    const int32_t i = cid[si];
+    const float4 qi = cposf[si];
+    int len1 = 0;  // the same in the 32 lanes of a half
+    for (int c0 = 0; c0 < nc; c0 += 32) {
+      bool pass = false;
+      if (act && c0 + hl < nc && c0 + hl != si) {
+        const float4 qj = cposf[c0 + hl];
+        float dx = qj.x - qi.x, dy = qj.y - qi.y, dz = qj.z - qi.z;
+        dx -= pfl.L[0] * rintf(dx * pfl.invL[0]);
+        dy -= pfl.L[1] * rintf(dy * pfl.invL[1]);
+        dz -= pfl.L[2] * rintf(dz * pfl.invL[2]);
+        pass = dx * dx + dy * dy + dz * dz <= pfl.r2;
+      }
+      const uint32_t m = (uint32_t)(__ballot(pass) >> (32 * hw));
+      const int r = len1 + __popc(m & hbelow);
+      if (pass && r < CR_KMAX) row[r] = c0 + hl;
+      len1 += __popc(m);
+    }
+    if (len1 > CR_KMAX) {
+      if (hl == 0) atomicOr(flag, 8);
+      len1 = CR_KMAX;
+    }
+    __builtin_amdgcn_wave_barrier();  // row stores before the loads below
+    const int lmax = max(len1, __shfl_xor(len1, 32, 64));
+    double4 pi = make_double4(0, 0, 0, 0);
+    if (act) pi = pw[i];
+    int len = 0;
+    for (int e0 = 0; e0 < lmax; e0 += 32) {
+      bool hit = false;
+      int32_t j = -1;
+      if (e0 + hl < len1) {
+        j = cid[row[e0 + hl]];
+        const double4 pj = pw[j];
+        hit = ipg_pe_sumsq(g.box, pi.x, pi.y, pi.z, pj.x, pj.y, pj.z) <= smax && snap_cut_ok(g.box, pi, pj, sc);
+      }
+      const uint32_t m = (uint32_t)(__ballot(hit) >> (32 * hw));
+      const int r = len + __popc(m & hbelow);
+      if (hit && r < K) ell[(int64_t)i * K + r] = j;
+      len += __popc(m);
+    }
+    if (act && hl == 0) {
+      if (len > K) atomicOr(flag, 8);
+      cnt[i] = len < K ? len : K;
+    }
+    __builtin_amdgcn_wave_barrier();  // loads done before the next rows are written
+  }
+}
+
 // In-segment rank sort: out[off + rank] = in[q], rank = number of smaller
 // entries of the same segment (values within a segment are distinct).
 __global__ void k_seg_rank_sort(int64_t nseg, const int64_t* __restrict__ off, const int32_t* __restrict__ in,
@@ -582,7 +754,7 @@ DevSystem upload_system(ipg_ctx* ctx, const ipg_system* s, const char* tag) {
 template <typename Alloc>
 static DevList build_list_impl(ipg_ctx* ctx, const DevSystem& s, double rcut, int kind, bool sorted, bool want_disp,
                                int64_t cap, int32_t* capflag, const SnapCutDev* sc, Alloc&& alloc, int ellK = 0,
-                               bool clear_flag = false, bool defer = false) {
+                               bool clear_flag = false, bool defer = false, bool binned = false) {
   IPG_REQUIRE(std::isfinite(rcut) && rcut > 0, "rcutoff must be finite and > 0");
   IPG_REQUIRE(!clear_flag || capflag, "internal: flag clear without a flag");
   DevList L;
@@ -608,20 +780,56 @@ static DevList build_list_impl(ipg_ctx* ctx, const DevSystem& s, double rcut, in
   const double err = 8.0 * ext * std::ldexp(1.0, -23) + 1e-6 * rcut;
   const double rr = rcut + 2.0 * std::sqrt(3.0) * err;
   pf.r2 = (float)(rr * rr * (1.0 + 1e-5));
-  int32_t* cellof = scratch_t<int32_t>(ctx, "nl_cellof", s.n);
-  int32_t* ccount = scratch_t<int32_t>(ctx, "nl_ccount", 2 * g.ncell + 2);
-  int32_t* cfill = ccount + g.ncell + 1;
-  int64_t* cstart = scratch_t<int64_t>(ctx, "nl_cstart", g.ncell + 1);
-  int32_t* ctmp = scratch_t<int32_t>(ctx, "nl_ctmp", s.n);
-  int32_t* catom = scratch_t<int32_t>(ctx, "nl_catom", s.n);
-  double4* spos = scratch_t<double4>(ctx, "nl_spos", s.n);
-  float4* sposf = scratch_t<float4>(ctx, "nl_sposf", s.n);
+  SnapCutDev scd{};
+  if (sc) scd = *sc;
+  // expected neighbors per atom at the grid's mean density
+  double vol = (double)g.ncell;
+  for (int d = 0; d < 3; d++) vol *= g.w[d];
+  const double expect = std::min((double)s.n / vol * 4.18879020478639 * rcut * rcut * rcut, (double)s.n);
+  if (ellK < 0) ellK = (int)std::min<double>(4096.0, std::ceil((1.5 * expect + 32.0) / 32.0) * 32.0);
+  const bool ell_mode = ellK > 0 && (sc ? sorted : !sorted);
+  L.ellK_req = ell_mode ? ellK : 0;
+  int32_t* ell = ell_mode ? scratch_t<int32_t>(ctx, "nl_ell", (size_t)s.n * ellK) : nullptr;
   int32_t* cnt = scratch_t<int32_t>(ctx, "nl_cnt", s.n);
   const unsigned gb = grid_for(s.n, 256);
+  // small periodic SNAP lists: cell-pair rows (every atom's cell scanned by
+  // every cell's block: n * ncell bounded; at most 24 atoms per cell on
+  // average, so the LDS lists of a cell and its neighborhood rarely fill)
+  bool packed = !binned && sc && ell_mode && ellK <= CR_KMAX && s.n <= CR_UMAX * 1024 &&
+                g.ncell <= CR_MAXCELL && s.n * g.ncell <= ((int64_t)1 << 25) && s.n <= 24 * g.ncell;
+  for (int d = 0; d < 3; d++) packed = packed && s.box.per[d] && g.nc[d] >= 3 && g.nc[d] <= 1023;
+  if (packed) {
+    IPG_REQUIRE(capflag != nullptr, "internal: row-buffer scan without a flag");
+    int32_t* cellc = scratch_t<int32_t>(ctx, "nl_cellc", (s.n + 3) / 4 * 4);  // read 4 at a time
+    double4* pw = scratch_t<double4>(ctx, "nl_pw", s.n);
+    float4* pfw = scratch_t<float4>(ctx, "nl_pf", s.n);
+    launch(ctx, "cell_pack", [&] {
+      k_cell_pack<<<gb, 256, 0, ctx->stream>>>(s.n, s.pos, s.species, g, cellc, pw, pfw,
+                                               clear_flag ? capflag : nullptr);
+    });
+    launch(ctx, "nlist_rows", [&] {
+      k_snapc_cell_rows<<<(unsigned)g.ncell, 256, 0, ctx->stream>>>((int32_t)s.n, g, pf, (const int4*)cellc, pw, pfw,
+                                                                    smax, scd, ellK, ell, cnt, capflag);
+    });
+  }
+  auto rebuild_binned = [&] {
+    return build_list_impl(ctx, s, rcut, kind, sorted, want_disp, cap, capflag, sc, alloc, ellK, clear_flag, defer,
+                           true);
+  };
+  int32_t* cellof = packed ? nullptr : scratch_t<int32_t>(ctx, "nl_cellof", s.n);
+  int32_t* ccount = packed ? nullptr : scratch_t<int32_t>(ctx, "nl_ccount", 2 * g.ncell + 2);
+  int32_t* cfill = packed ? nullptr : ccount + g.ncell + 1;
+  int64_t* cstart = packed ? nullptr : scratch_t<int64_t>(ctx, "nl_cstart", g.ncell + 1);
+  int32_t* ctmp = packed ? nullptr : scratch_t<int32_t>(ctx, "nl_ctmp", s.n);
+  int32_t* catom = packed ? nullptr : scratch_t<int32_t>(ctx, "nl_catom", s.n);
+  double4* spos = packed ? nullptr : scratch_t<double4>(ctx, "nl_spos", s.n);
+  float4* sposf = packed ? nullptr : scratch_t<float4>(ctx, "nl_sposf", s.n);
   const size_t cells_lds = sizeof(int32_t) * (size_t)(s.n + g.ncell + 1);
   // the ordering block is latency-bound on one CU: 8192 atoms took 27 us
   // there against about 20 us for the four kernels, 1000 atoms a few us
-  if (s.n <= 4096 && cells_lds <= 48 * 1024) {
+  if (packed) {
+    // rows written above
+  } else if (s.n <= 4096 && cells_lds <= 48 * 1024) {
     double4* pw = scratch_t<double4>(ctx, "nl_pw", s.n);
     float4* pfw = scratch_t<float4>(ctx, "nl_pf", s.n);
     launch(ctx, "cell_assign", [&] {
@@ -661,16 +869,6 @@ static DevList build_list_impl(ipg_ctx* ctx, const DevSystem& s, double rcut, in
   const double cand = (double)s.n / (double)g.ncell * (g.full[0] ? g.nc[0] : std::min(3, g.nc[0]));
   const int LP = cand >= 80 ? 64 : (cand >= 18 ? 32 : (cand >= 9 ? 16 : 8));
   const unsigned gl = grid_for(s.n * LP, 256);
-  SnapCutDev scd{};
-  if (sc) scd = *sc;
-  // expected neighbors per atom at the grid's mean density
-  double vol = (double)g.ncell;
-  for (int d = 0; d < 3; d++) vol *= g.w[d];
-  const double expect = std::min((double)s.n / vol * 4.18879020478639 * rcut * rcut * rcut, (double)s.n);
-  if (ellK < 0) ellK = (int)std::min<double>(4096.0, std::ceil((1.5 * expect + 32.0) / 32.0) * 32.0);
-  const bool ell_mode = ellK > 0 && (sc ? sorted : !sorted);
-  L.ellK_req = ell_mode ? ellK : 0;
-  int32_t* ell = ell_mode ? scratch_t<int32_t>(ctx, "nl_ell", (size_t)s.n * ellK) : nullptr;
   auto scan = [&](auto lp, auto mode, int64_t* offp, int32_t* nbrp) {
     constexpr int LPc = decltype(lp)::value;
     constexpr int M = decltype(mode)::value;
@@ -687,7 +885,9 @@ static DevList build_list_impl(ipg_ctx* ctx, const DevSystem& s, double rcut, in
     else if (LP == 16) scan(std::integral_constant<int, 16>(), mode, offp, nbrp);
     else scan(std::integral_constant<int, 8>(), mode, offp, nbrp);
   };
-  if (ell_mode) {
+  if (packed) {
+    // rows and counts written by the cell-pair kernel
+  } else if (ell_mode) {
     IPG_REQUIRE(capflag != nullptr, "internal: row-buffer scan without a flag");
     launch(ctx, "nlist_count", [&] { dispatch(std::integral_constant<int, SCAN_ELL>(), nullptr, ell); });
   } else {
@@ -730,8 +930,10 @@ static DevList build_list_impl(ipg_ctx* ctx, const DevSystem& s, double rcut, in
       L.ellK = ellK;
       L.ell = ell;
       L.cnt = cnt;
+      L.ell_sorted = packed;
       return L;
     }
+    if (packed) return rebuild_binned();  // a row, a cell or a neighborhood overflowed: the binned scan
     launch(ctx, "nlist_count", [&] { dispatch(std::integral_constant<int, SCAN_COUNT>(), nullptr, nullptr); });
     np = exclusive_scan(ctx, cnt, s.n, L.off, true);
     std::vector<int32_t> hc((size_t)s.n);
@@ -967,6 +1169,61 @@ __global__ __launch_bounds__(256) void k_snapc_finish(int64_t n, int K, const in
   }
 }
 
+// The same from rows already sorted by j (the cell-pair kernel's): no rank
+// pass, and with rev the reverse pair of q = (i -> j) is found by binary
+// search in j's row of the row buffer (complete when this kernel starts):
+// rev[q] = off[j] + the position of i there.  A pair whose reverse is
+// missing (r.r within an ulp of the cutoff under wrapping, or j's row cut
+// by a capacity) points at itself and sets bit 4 of *flag.
+template <bool REV>
+__global__ __launch_bounds__(256) void k_snapc_finish_sorted(int64_t n, int K, const int32_t* __restrict__ ell,
+                                                             const int32_t* __restrict__ cnt,
+                                                             const int64_t* __restrict__ off,
+                                                             const double* __restrict__ pos, IpgBox box,
+                                                             int32_t* __restrict__ nbr, int32_t* __restrict__ center,
+                                                             double* __restrict__ rvec, double* __restrict__ R,
+                                                             int32_t* __restrict__ rev, int32_t* __restrict__ flag) {
+#pragma clang fp contract(off)
+  constexpr int LP = 32;
+  const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t i = gt / LP;
+  const int l = (int)(gt % LP);
+  if (i >= n) return;
+  const int c = min(cnt[i], K);
+  const int64_t a = off[i], b = off[i + 1];
+  const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+  for (int k = l; k < c; k += LP) {
+    const int32_t j = ell[i * K + k];
+    const int64_t q = a + k;
+    if (q >= b) break;  // row cut by a capacity (flagged by the scan)
+    nbr[q] = j;
+    center[q] = (int32_t)i;
+    const double rx = -1.0 * ipg_disp1(xi, pos[3 * j], box.L[0], box.per[0]);
+    const double ry = -1.0 * ipg_disp1(yi, pos[3 * j + 1], box.L[1], box.per[1]);
+    const double rz = -1.0 * ipg_disp1(zi, pos[3 * j + 2], box.L[2], box.per[2]);
+    rvec[3 * q] = rx;
+    rvec[3 * q + 1] = ry;
+    rvec[3 * q + 2] = rz;
+    if (R) R[q] = sqrt(rx * rx + ry * ry + rz * rz);
+    if constexpr (REV) {
+      const int32_t* rj = ell + (int64_t)j * K;
+      const int cj = min(cnt[j], K);
+      int lo = 0, hi = cj;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (rj[mid] < i) lo = mid + 1;
+        else hi = mid;
+      }
+      if (lo < cj && rj[lo] == i && off[j] + lo < off[j + 1]) {
+        rev[q] = (int32_t)(off[j] + lo);
+      } else {
+        rev[q] = (int32_t)q;
+        atomicOr(flag, 4);
+      }
+    }
+  }
+}
+
 // rev[q]: the pair j -> i of pair q = (i -> j), by binary search in j's
 // sorted row (the rows are complete once the finish kernel has run).  A
 // pair whose reverse is missing (r.r within an ulp of the cutoff under
@@ -1019,7 +1276,17 @@ SnapListDev build_snap_list(ipg_ctx* ctx, const DevSystem& s, const int32_t* d_a
     out.R = (need & SNAPL_R) ? scratch_t<double>(ctx, (t + "_R").c_str(), npa) : nullptr;
     out.rev = (need & SNAPL_SYM) ? scratch_t<int32_t>(ctx, (t + "_rev").c_str(), npa) : nullptr;
     IPG_REQUIRE(!out.rev || flag, "internal: reverse-pair search without a flag");
-    if (n > 0 && L.npairs > 0 && L.ellK > 0) {
+    if (n > 0 && L.npairs > 0 && L.ellK > 0 && L.ell_sorted) {
+      launch(ctx, "snap_list_finish", [&] {
+        const unsigned gf = grid_for(n * 32, 256);
+        if (out.rev)
+          k_snapc_finish_sorted<true><<<gf, 256, 0, ctx->stream>>>(n, L.ellK, L.ell, L.cnt, L.off, s.pos, s.box, L.nbr,
+                                                                  out.center, out.rvec, out.R, out.rev, flag);
+        else
+          k_snapc_finish_sorted<false><<<gf, 256, 0, ctx->stream>>>(n, L.ellK, L.ell, L.cnt, L.off, s.pos, s.box,
+                                                                   L.nbr, out.center, out.rvec, out.R, nullptr, flag);
+      });
+    } else if (n > 0 && L.npairs > 0 && L.ellK > 0) {
       const int K = L.ellK;
       const int LP = K >= 128 ? 64 : 32;
       const size_t sm = sizeof(int32_t) * (size_t)(256 / LP) * K;

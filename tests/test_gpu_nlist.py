@@ -167,3 +167,57 @@ def test_snap_list_unequal_radii(ipg, orc, mode):
     # result: exercises the as-is element lookups of utl.jl:70,86
     npairs = _snap_case(ipg, orc, mode, [0.3, 1.5, 0.9], 3, 11, rcutfac=1.0, dense=True)
     assert npairs > 0
+
+
+def _snap_list_check(ipg, orc, X, L, syms, names, radii, rcutfac, twojmax=4):
+    s = ipg.AtomsSystem(X, L, (True,) * 3, syms, unit="angstrom")
+    ns = len(names)
+    sn = ipg.snap.SNAP(len(X), twojmax, names, rcutfac, 0.0, 0.99363, radii, [1.0] * ns, list_mode="cutoff")
+    g = ipg.snap.snap_neighborlist(s, sn)
+    o = orc.Snap(twojmax, ns, rcutfac, 0.0, 0.99363, radii, [1.0] * ns)
+    sp = np.array([names.index(x) for x in syms], dtype=np.int32)
+    uniq = s.unique_symbols()
+    ap = np.array([uniq.index(x) for x in syms], dtype=np.int32)
+    off, j, R, r = o.neighborlist(X, L, sp, appear=ap, mode=orc.SNAP_LIST_CUTOFF)
+    np.testing.assert_array_equal(g.offsets, off)
+    np.testing.assert_array_equal(g.j, j)
+    np.testing.assert_array_equal(g.r, r)
+    np.testing.assert_allclose(g.R, R, rtol=2e-16, atol=0)
+    return np.diff(off)
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_snap_list_cell_rows(ipg, orc, seed):
+    """CUTOFF-mode SNAP lists of periodic systems with at least 3 cells a
+    side and a few atoms per cell take the cell-pair row kernel (rows
+    gathered per home cell, already in j order).  Two species with unequal
+    radii, so the species-pair cutoffs drop part of the query-radius hits;
+    positions shifted by whole box lengths on some atoms (the exact test
+    works on the unwrapped coordinates)."""
+    from ipgpu import workloads
+
+    rng = np.random.default_rng(seed)
+    X, L = workloads.bcc_positions(8, 8, 8, 3.16)
+    X = X + rng.normal(scale=0.15, size=X.shape)
+    X[::7] += np.array(L) * rng.integers(-2, 3, size=(len(X[::7]), 3))
+    names = ["W", "Mo"]
+    syms = [names[k] for k in rng.integers(0, 2, len(X))]
+    cnt = _snap_list_check(ipg, orc, X, L, syms, names, [1.0, 0.8], 2.4)
+    assert cnt.min() > 5 and cnt.max() < 60
+
+
+def test_snap_list_cell_rows_overflow_falls_back(ipg, orc):
+    """A dense cluster of 70 atoms inside one cell overfills the cell-pair
+    kernel's per-cell and per-row buffers; the list is rebuilt by the binned
+    scan and still equals the oracle's."""
+    from ipgpu import workloads
+
+    rng = np.random.default_rng(5)
+    X, L = workloads.bcc_positions(8, 8, 8, 3.16)
+    X = X + rng.normal(scale=0.1, size=X.shape)
+    cl = np.array([12.0, 12.0, 12.0]) + rng.random((70, 3)) * 3.0
+    X = np.vstack([X, cl])
+    names = ["W", "Mo"]
+    syms = [names[k] for k in rng.integers(0, 2, len(X))]
+    cnt = _snap_list_check(ipg, orc, X, L, syms, names, [1.0, 0.8], 2.4)
+    assert cnt.max() > 96
