@@ -427,56 +427,72 @@ __global__ __launch_bounds__(256) void k_list_scan(int64_t n, Grid g, Prefilter 
 }
 
 // Cell-pair rows for small periodic SNAP lists (every dimension periodic
-// with 3..1023 cells, so the 27 cells around a cell are distinct).  No sort
-// by cell: k_cell_pack writes each atom's cell, its exact coordinates with
-// the species index in w, and its FP32 wrapped coordinates.  Then one block
-// per home cell reads every atom's cell (thread t a contiguous range, all
-// loads in flight at once), tests it against a bitmap of the 27 cells
-// around the home cell, and compacts the candidates into LDS in atom order
-// (block prefix sum).  Half-waves take the home cell's atoms: the FP32
-// prefilter of k_list_scan over every candidate, then the exact tests
+// with 4..1023 cells: the 27 cells around a cell are distinct and lie
+// within 1.5 cells of its center, less than half a box length).  No sort by
+// cell: k_cell_pack writes each atom's cell (16 bits), its coordinates and
+// species (exact) and its FP32 wrapped coordinates.  Then one block per
+// home cell reads every atom's cell (coalesced; thread t holds atoms
+// 2048 u + 8 t + e), tests it against a bitmap of the 27 cells around the
+// home cell, and compacts the candidates in atom order (a block prefix sum
+// of the per-u counts, packed four to a 64-bit word).  The candidates'
+// coordinates are staged in LDS: exact, and FP32 moved to the periodic
+// image nearest the home cell's center, so that the prefilter needs no
+// wrapping.  Half-waves take the home cell's atoms: the FP32 prefilter
+// (the bound of k_list_scan) over every candidate, then the exact tests
 // (PeriodicEuclidean and the species-pair cutoff) on the survivors, whose
 // order is kept, so each row comes out sorted by j and goes straight to
 // ell.  cnt[i] = row length (clamped to K, with bit 8 of *flag on
 // overflow); a cell holding more than CR_MAXH atoms, a neighborhood of
 // more than CR_MAXC, or more than CR_KMAX prefilter survivors also set
 // bit 8, and the caller falls back to the binned scan.
-constexpr int CR_MAXC = 768, CR_MAXH = 64, CR_KMAX = 128, CR_UMAX = 16, CR_MAXCELL = 16384;
+constexpr int CR_MAXC = 640, CR_MAXH = 64, CR_KMAX = 128, CR_UMAX = 8, CR_MAXCELL = 16384;
+constexpr int CR_MAXN = CR_UMAX * 8 * 256;  // atoms: 8 cells per 16-byte load, CR_UMAX loads per thread
 __global__ void k_cell_pack(int64_t n, const double* __restrict__ pos, const int32_t* __restrict__ species, Grid g,
-                            int32_t* __restrict__ cellc, double4* __restrict__ pw, float4* __restrict__ pf,
+                            uint16_t* __restrict__ cellc, double4* __restrict__ pw, float4* __restrict__ pf,
                             int32_t* __restrict__ clear) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (clear && i == 0) *clear = 0;
   if (i >= n) return;
   const double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
-  cellc[i] = (int32_t)(((int64_t)cell_coord(g, 2, z) * g.nc[1] + cell_coord(g, 1, y)) * g.nc[0] + cell_coord(g, 0, x));
-  pw[i] = make_double4(x, y, z, species ? (double)species[i] : 0.0);
+  cellc[i] = (uint16_t)(((int64_t)cell_coord(g, 2, z) * g.nc[1] + cell_coord(g, 1, y)) * g.nc[0] + cell_coord(g, 0, x));
+  const double sp = species ? (double)species[i] : 0.0;
+  pw[i] = make_double4(x, y, z, sp);
   double w[3] = {x, y, z};
   for (int d = 0; d < 3; d++) w[d] = w[d] - floor(w[d] / g.box.L[d]) * g.box.L[d];
-  pf[i] = make_float4((float)w[0], (float)w[1], (float)w[2], 0.f);
+  pf[i] = make_float4((float)w[0], (float)w[1], (float)w[2], (float)sp);  // w: species index (exact)
+}
+
+__device__ __forceinline__ uint32_t cr_field(uint64_t lo, uint64_t hi, int u) {  // 16-bit field u of (lo, hi)
+  return (uint32_t)(((u < 4 ? lo : hi) >> (16 * (u & 3))) & 0xffffu);
 }
 
 __global__ __launch_bounds__(256) void k_snapc_cell_rows(int32_t n, Grid g, Prefilter pfl,
-                                                         const int4* __restrict__ cellc4,
+                                                         const int4* __restrict__ cellc8,
                                                          const double4* __restrict__ pw,
                                                          const float4* __restrict__ pf, double smax, SnapCutDev sc,
                                                          int K, int32_t* __restrict__ ell, int32_t* __restrict__ cnt,
                                                          int32_t* __restrict__ flag) {
   __shared__ int32_t cid[CR_MAXC];
-  __shared__ float4 cposf[CR_MAXC];
+  __shared__ float4 cposf[CR_MAXC];  // FP32 image nearest the home cell's center; w: species
+  __shared__ double cx[CR_MAXC], cy[CR_MAXC], cz[CR_MAXC];
   __shared__ uint32_t bitmap[CR_MAXCELL / 32];
   __shared__ int32_t hid[CR_MAXH];  // LDS slots of the home cell's atoms
   __shared__ int32_t rows[8][CR_KMAX];
-  __shared__ int32_t wtot[4];
+  __shared__ uint64_t wtot[2][4];
   __shared__ int32_t nhome;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const int h = blockIdx.x;
+  const int hx = h % g.nc[0], hy = (h / g.nc[0]) % g.nc[1], hz = h / (g.nc[0] * g.nc[1]);
+  const int nq = (n + 7) >> 3;  // 16-byte loads of 8 cells
+  int4 v[CR_UMAX];
+#pragma unroll
+  for (int u = 0; u < CR_UMAX; u++)
+    if (t + 256 * u < nq) v[u] = cellc8[t + 256 * u];
   const int nwords = (int)((g.ncell + 31) >> 5);
   for (int w = t; w < nwords; w += 256) bitmap[w] = 0u;
   if (t == 0) nhome = 0;
   __syncthreads();
   if (t < 27) {
-    const int hx = h % g.nc[0], hy = (h / g.nc[0]) % g.nc[1], hz = h / (g.nc[0] * g.nc[1]);
     int x = hx + t % 3 - 1, y = hy + (t / 3) % 3 - 1, z = hz + t / 9 - 1;
     x += x < 0 ? g.nc[0] : (x >= g.nc[0] ? -g.nc[0] : 0);
     y += y < 0 ? g.nc[1] : (y >= g.nc[1] ? -g.nc[1] : 0);
@@ -484,58 +500,95 @@ __global__ __launch_bounds__(256) void k_snapc_cell_rows(int32_t n, Grid g, Pref
     const int c = (z * g.nc[1] + y) * g.nc[0] + x;
     atomicOr(&bitmap[c >> 5], 1u << (c & 31));
   }
-  // thread t reads the cells of atoms [4 per t, 4 (per t + per)), per int4
-  // loads of 4 atoms; bit 4u+e of the masks is atom 4 (per t + u) + e
-  const int nq = (n + 3) >> 2;
-  const int per = (nq + 255) >> 8;
-  int4 v[CR_UMAX];
-#pragma unroll
-  for (int u = 0; u < CR_UMAX; u++)
-    if (u < per && per * t + u < nq) v[u] = cellc4[per * t + u];
   __syncthreads();  // bitmap complete
+  // bit 8u+e of the masks: atom 2048 u + 8 t + e
   uint64_t cm = 0, hm = 0;
 #pragma unroll
   for (int u = 0; u < CR_UMAX; u++) {
-    if (u < per && per * t + u < nq) {
-      const int vv[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+    if (t + 256 * u < nq) {
+      const uint32_t vv[4] = {(uint32_t)v[u].x, (uint32_t)v[u].y, (uint32_t)v[u].z, (uint32_t)v[u].w};
 #pragma unroll
-      for (int e = 0; e < 4; e++) {
-        const bool in = 4 * (per * t + u) + e < n;
-        const int c = in ? vv[e] : 0;  // the padding after atom n - 1 is not written
-        const uint64_t bit = 1ull << (4 * u + e);
+      for (int e = 0; e < 8; e++) {
+        const bool in = 8 * (t + 256 * u) + e < n;
+        const int c = in ? (int)((vv[e >> 1] >> (16 * (e & 1))) & 0xffffu) : 0;  // padding after atom n - 1
+        const uint64_t bit = 1ull << (8 * u + e);
         if (in && ((bitmap[c >> 5] >> (c & 31)) & 1u)) cm |= bit;
         if (in && c == h) hm |= bit;
       }
     }
   }
-  // block exclusive prefix of the candidate counts (thread order = atom order)
-  const int mine = __popcll(cm);
-  int incl = mine;
+  // per-u candidate counts, four 16-bit fields per word; block prefix sums
+  uint64_t p0 = 0, p1 = 0;
+#pragma unroll
+  for (int u = 0; u < CR_UMAX; u++) {
+    const uint64_t c = (uint64_t)__popcll((cm >> (8 * u)) & 0xffull);
+    if (u < 4) p0 |= c << (16 * u);
+    else p1 |= c << (16 * (u - 4));
+  }
+  uint64_t i0 = p0, i1 = p1;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
-    const int y = __shfl_up(incl, o, 64);
-    if (lane >= o) incl += y;
+    const uint64_t y0 = __shfl_up(i0, o, 64), y1 = __shfl_up(i1, o, 64);
+    if (lane >= o) {
+      i0 += y0;
+      i1 += y1;
+    }
   }
-  if (lane == 63) wtot[wv] = incl;
+  if (lane == 63) {
+    wtot[0][wv] = i0;
+    wtot[1][wv] = i1;
+  }
   __syncthreads();
-  int base = incl - mine;
-  for (int w = 0; w < wv; w++) base += wtot[w];
-  const int nc = wtot[0] + wtot[1] + wtot[2] + wtot[3];
+  uint64_t e0 = i0 - p0, e1 = i1 - p1, t0 = 0, t1 = 0;
+  for (int w = 0; w < 4; w++) {
+    if (w < wv) {
+      e0 += wtot[0][w];
+      e1 += wtot[1][w];
+    }
+    t0 += wtot[0][w];
+    t1 += wtot[1][w];
+  }
+  int nc = 0;
+#pragma unroll
+  for (int u = 0; u < CR_UMAX; u++) nc += (int)cr_field(t0, t1, u);
   if (nc > CR_MAXC) {
     if (t == 0) atomicOr(flag, 8);
     return;  // uniform: nc is the block total
   }
-  while (cm) {
-    const int b = __builtin_ctzll(cm);
-    const int32_t k = 4 * (per * t + (b >> 2)) + (b & 3);
-    cid[base] = k;
-    cposf[base] = pf[k];
-    if ((hm >> b) & 1) {
-      const int sl = atomicAdd(&nhome, 1);
-      if (sl < CR_MAXH) hid[sl] = base;
+  int ou = 0;  // candidates of the slices before u
+#pragma unroll
+  for (int u = 0; u < CR_UMAX; u++) {
+    uint32_t m = (uint32_t)((cm >> (8 * u)) & 0xffull);
+    const uint32_t mh = (uint32_t)((hm >> (8 * u)) & 0xffull);
+    int base = ou + (int)cr_field(e0, e1, u);
+    while (m) {
+      const int e = __builtin_ctz(m);
+      cid[base] = 8 * (t + 256 * u) + e;
+      if ((mh >> e) & 1u) {
+        const int sl = atomicAdd(&nhome, 1);
+        if (sl < CR_MAXH) hid[sl] = base;
+      }
+      base++;
+      m &= m - 1;
     }
-    base++;
-    cm &= cm - 1;
+    ou += (int)cr_field(t0, t1, u);
+  }
+  __syncthreads();
+  {
+    // FP32 coordinates moved to the image nearest the home cell's center
+    const float hc[3] = {(float)((hx + 0.5) * g.w[0]), (float)((hy + 0.5) * g.w[1]), (float)((hz + 0.5) * g.w[2])};
+    for (int k = t; k < nc; k += 256) {  // one round of loads for the whole neighborhood
+      const int32_t j = cid[k];
+      const double4 p = pw[j];
+      float4 q = pf[j];
+      q.x += pfl.L[0] * rintf((hc[0] - q.x) * pfl.invL[0]);
+      q.y += pfl.L[1] * rintf((hc[1] - q.y) * pfl.invL[1]);
+      q.z += pfl.L[2] * rintf((hc[2] - q.z) * pfl.invL[2]);
+      cposf[k] = q;
+      cx[k] = p.x;
+      cy[k] = p.y;
+      cz[k] = p.z;
+    }
   }
   __syncthreads();
   const int nh = nhome;
@@ -557,10 +610,7 @@ __global__ __launch_bounds__(256) void k_snapc_cell_rows(int32_t n, Grid g, Pref
       bool pass = false;
       if (act && c0 + hl < nc && c0 + hl != si) {
         const float4 qj = cposf[c0 + hl];
-        float dx = qj.x - qi.x, dy = qj.y - qi.y, dz = qj.z - qi.z;
-        dx -= pfl.L[0] * rintf(dx * pfl.invL[0]);
-        dy -= pfl.L[1] * rintf(dy * pfl.invL[1]);
-        dz -= pfl.L[2] * rintf(dz * pfl.invL[2]);
+        const float dx = qj.x - qi.x, dy = qj.y - qi.y, dz = qj.z - qi.z;
         pass = dx * dx + dy * dy + dz * dz <= pfl.r2;
       }
       const uint32_t m = (uint32_t)(__ballot(pass) >> (32 * hw));
@@ -574,15 +624,15 @@ __global__ __launch_bounds__(256) void k_snapc_cell_rows(int32_t n, Grid g, Pref
     }
     __builtin_amdgcn_wave_barrier();  // row stores before the loads below
     const int lmax = max(len1, __shfl_xor(len1, 32, 64));
-    double4 pi = make_double4(0, 0, 0, 0);
-    if (act) pi = pw[i];
+    const double4 pi = make_double4(cx[si], cy[si], cz[si], (double)qi.w);
     int len = 0;
     for (int e0 = 0; e0 < lmax; e0 += 32) {
       bool hit = false;
       int32_t j = -1;
       if (e0 + hl < len1) {
-        j = cid[row[e0 + hl]];
-        const double4 pj = pw[j];
+        const int sj = row[e0 + hl];
+        j = cid[sj];
+        const double4 pj = make_double4(cx[sj], cy[sj], cz[sj], (double)cposf[sj].w);
         hit = ipg_pe_sumsq(g.box, pi.x, pi.y, pi.z, pj.x, pj.y, pj.z) <= smax && snap_cut_ok(g.box, pi, pj, sc);
       }
       const uint32_t m = (uint32_t)(__ballot(hit) >> (32 * hw));
@@ -793,14 +843,14 @@ static DevList build_list_impl(ipg_ctx* ctx, const DevSystem& s, double rcut, in
   int32_t* cnt = scratch_t<int32_t>(ctx, "nl_cnt", s.n);
   const unsigned gb = grid_for(s.n, 256);
   // small periodic SNAP lists: cell-pair rows (every atom's cell scanned by
-  // every cell's block: n * ncell bounded; at most 24 atoms per cell on
+  // every cell's block: n * ncell bounded; at most 20 atoms per cell on
   // average, so the LDS lists of a cell and its neighborhood rarely fill)
-  bool packed = !binned && sc && ell_mode && ellK <= CR_KMAX && s.n <= CR_UMAX * 1024 &&
-                g.ncell <= CR_MAXCELL && s.n * g.ncell <= ((int64_t)1 << 25) && s.n <= 24 * g.ncell;
-  for (int d = 0; d < 3; d++) packed = packed && s.box.per[d] && g.nc[d] >= 3 && g.nc[d] <= 1023;
+  bool packed = !binned && sc && ell_mode && ellK <= CR_KMAX && s.n <= CR_MAXN &&
+                g.ncell <= CR_MAXCELL && s.n * g.ncell <= ((int64_t)1 << 25) && s.n <= 20 * g.ncell;
+  for (int d = 0; d < 3; d++) packed = packed && s.box.per[d] && g.nc[d] >= 4 && g.nc[d] <= 1023;
   if (packed) {
     IPG_REQUIRE(capflag != nullptr, "internal: row-buffer scan without a flag");
-    int32_t* cellc = scratch_t<int32_t>(ctx, "nl_cellc", (s.n + 3) / 4 * 4);  // read 4 at a time
+    uint16_t* cellc = scratch_t<uint16_t>(ctx, "nl_cellc", (s.n + 7) / 8 * 8);  // read 8 at a time
     double4* pw = scratch_t<double4>(ctx, "nl_pw", s.n);
     float4* pfw = scratch_t<float4>(ctx, "nl_pf", s.n);
     launch(ctx, "cell_pack", [&] {
@@ -808,8 +858,8 @@ static DevList build_list_impl(ipg_ctx* ctx, const DevSystem& s, double rcut, in
                                                clear_flag ? capflag : nullptr);
     });
     launch(ctx, "nlist_rows", [&] {
-      k_snapc_cell_rows<<<(unsigned)g.ncell, 256, 0, ctx->stream>>>((int32_t)s.n, g, pf, (const int4*)cellc, pw, pfw,
-                                                                    smax, scd, ellK, ell, cnt, capflag);
+      k_snapc_cell_rows<<<(unsigned)g.ncell, 256, 0, ctx->stream>>>((int32_t)s.n, g, pf, (const int4*)cellc, pw,
+                                                                     pfw, smax, scd, ellK, ell, cnt, capflag);
     });
   }
   auto rebuild_binned = [&] {
