@@ -551,7 +551,21 @@ __global__ __launch_bounds__(256) void k_snapc_cell_rows(int32_t n, Grid g, Pref
   int nc = 0;
 #pragma unroll
   for (int u = 0; u < CR_UMAX; u++) nc += (int)cr_field(t0, t1, u);
+  // a cell that overflows the LDS lists leaves its atoms with empty rows
+  // (and bit 8 set), so no later kernel of a captured step reads a stale row
+  auto empty_rows = [&] {
+#pragma unroll
+    for (int u = 0; u < CR_UMAX; u++) {
+      uint32_t mh = (uint32_t)((hm >> (8 * u)) & 0xffull);
+      while (mh) {
+        const int e = __builtin_ctz(mh);
+        cnt[8 * (t + 256 * u) + e] = 0;
+        mh &= mh - 1;
+      }
+    }
+  };
   if (nc > CR_MAXC) {
+    empty_rows();
     if (t == 0) atomicOr(flag, 8);
     return;  // uniform: nc is the block total
   }
@@ -593,6 +607,7 @@ __global__ __launch_bounds__(256) void k_snapc_cell_rows(int32_t n, Grid g, Pref
   __syncthreads();
   const int nh = nhome;
   if (nh > CR_MAXH) {
+    empty_rows();
     if (t == 0) atomicOr(flag, 8);
     return;
   }

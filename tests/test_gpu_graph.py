@@ -145,3 +145,25 @@ def test_prof_select_times_only_the_named_kernels(ipg):
         device.prof_enable(False)
         device.prof_select(None)
         device.set_graphs(True)
+
+
+@pytest.mark.parametrize("path", ["adjoint_fp32", "plain"])
+def test_graph_replay_cell_overflow(ipg, path):
+    """1024 atoms, five cells a side: the SNAP list takes the cell-pair row
+    kernel.  Moving 70 atoms in place into one cell overfills that kernel's
+    per-cell buffers on the next replay; the replay's flag word sends the
+    call to the exact (binned) rebuild, whose result equals the uncaptured
+    call's bit for bit, and the replays after it stay correct."""
+    s, sn, d, dsys = setup(ipg, ncell=8, seed=5)
+    ref = uncaptured(ipg, sn, d, dsys, path)
+    assert same(step(ipg, sn, d, dsys, path), ref)  # exact run, then capture
+    assert same(step(ipg, sn, d, dsys, path), ref)  # replay
+    rng = np.random.default_rng(9)
+    X2 = s.positions.copy()
+    X2[:70] = np.array([11.0, 11.0, 11.0]) + rng.random((70, 3)) * 3.0
+    d["pos"].upload(X2)
+    got = step(ipg, sn, d, dsys, path)
+    ref2 = uncaptured(ipg, sn, d, dsys, path)
+    assert same(got, ref2)
+    assert sn.graph_info()["overflows"] >= 1
+    assert same(step(ipg, sn, d, dsys, path), ref2)
