@@ -107,7 +107,8 @@ def test_large_periodic_lattice(ipg, orc):
 
 
 @pytest.mark.parametrize("n,per", [(3000, (True, True, True)), (9000, (True, True, True)),
-                                   (9000, (True, False, True)), (20000, (True, True, True))])
+                                   (9000, (True, False, True)), (20000, (True, True, True)),
+                                   (40000, (True, True, True)), (40000, (False, True, True))])
 def test_binning_paths_by_size(ipg, orc, n, per):
     """The cell binning runs in two launches up to 4096 atoms, three up to
     16384 and five above; every path gives the oracle's lists exactly (half
@@ -122,6 +123,19 @@ def test_binning_paths_by_size(ipg, orc, n, per):
         b = orc.neighborlist(pos, Lo, 5.0, kind=orc.LIST_FULL if full else orc.LIST_HALF, cells=True)
         assert_same(a, b)
         assert len(a[1]) > (10 if full else 5) * n
+
+
+def test_thin_periodic_slab(ipg, orc):
+    """40,000 atoms in a periodic slab 8 A thick: one cell across it, fewer
+    than the 3 of the stencil, so the scan visits that dimension's cells
+    once each (minimum-image prefilter); the lists equal the oracle's."""
+    rng = np.random.default_rng(21)
+    box = [200.0, 200.0, 8.0]
+    pos = rng.random((40000, 3)) * np.array(box)
+    for full in (False, True):
+        a = gpu_list(ipg, pos, box, (True, True, True), 5.0, full=full)
+        b = orc.neighborlist(pos, box, 5.0, kind=orc.LIST_FULL if full else orc.LIST_HALF, cells=True)
+        assert_same(a, b)
 
 
 def _snap_case(ipg, orc, mode, radii, nspecies, seed, n=120, a=3.16, rcutfac=2.0, dense=False):
