@@ -19,7 +19,8 @@ struct Grid {
   IpgBox box;
   double lo[3], w[3];
   int nc[3];
-  int full[3];  // periodic dimension with < 3 cells: visit all of them once
+  int full[3];  // periodic dimension with fewer than 2 hw + 1 cells: visit all of them once
+  int hw;       // cells are at least rcut / hw wide; a cell's candidates lie within hw cells of it
   int64_t ncell;
 };
 
@@ -190,71 +191,29 @@ __global__ __launch_bounds__(CELLS_BLOCK) void k_cells_order(int32_t n, int32_t 
   }
 }
 
-// Candidate atoms of a cell: the 3x3x3 block of cells around it (all cells
-// of a periodic dimension with fewer than 3 cells, clipped at open
-// boundaries), each visited once.  Cells are numbered x fastest and atoms
-// are sorted by cell, so the cells x-1, x, x+1 of one (y, z) row hold one
-// contiguous range of atoms (two ranges where a periodic row wraps): f gets
-// 9 ranges instead of 27 cells.
-template <typename F>
-__device__ __forceinline__ void for_each_row(const Grid& g, int32_t cell, const int64_t* __restrict__ cstart,
-                                             F&& f) {
-  const int nx = g.nc[0];
-  const int cx = cell % nx, cy = (cell / nx) % g.nc[1], cz = cell / (nx * g.nc[1]);
-  const int ylo = g.full[1] ? 0 : cy - 1, yhi = g.full[1] ? g.nc[1] - 1 : cy + 1;
-  const int zlo = g.full[2] ? 0 : cz - 1, zhi = g.full[2] ? g.nc[2] - 1 : cz + 1;
-  for (int z = zlo; z <= zhi; z++) {
-    int zz = z;
-    if (!g.full[2]) {
-      if (g.box.per[2]) zz = (z + g.nc[2]) % g.nc[2];
-      else if (z < 0 || z >= g.nc[2]) continue;
-    }
-    for (int y = ylo; y <= yhi; y++) {
-      int yy = y;
-      if (!g.full[1]) {
-        if (g.box.per[1]) yy = (y + g.nc[1]) % g.nc[1];
-        else if (y < 0 || y >= g.nc[1]) continue;
-      }
-      const int64_t rb = ((int64_t)zz * g.nc[1] + yy) * nx;
-      if (g.full[0]) {
-        f(cstart[rb], cstart[rb + nx]);
-      } else if (g.box.per[0]) {  // nx >= 3 here
-        if (cx == 0) {
-          f(cstart[rb + nx - 1], cstart[rb + nx]);
-          f(cstart[rb], cstart[rb + 2]);
-        } else if (cx == nx - 1) {
-          f(cstart[rb + nx - 2], cstart[rb + nx]);
-          f(cstart[rb], cstart[rb + 1]);
-        } else {
-          f(cstart[rb + cx - 1], cstart[rb + cx + 2]);
-        }
-      } else {
-        const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < nx ? cx + 1 : nx - 1;
-        f(cstart[rb + x0], cstart[rb + x1 + 1]);
-      }
-    }
-  }
-}
-
-// Range r (0..17) of the candidates of a cell: row r/2 of the (y, z) rows
-// of for_each_row, run r%2 of that row (the second run exists only where a
-// periodic x row wraps).  Empty ranges come back as s0 == s1.
+// Candidate atoms of a cell: the cells within hw of it in each dimension
+// (all cells of a periodic dimension with fewer than 2 hw + 1 cells,
+// clipped at open boundaries), each visited once.  Cells are numbered x
+// fastest and atoms are sorted by cell, so the cells x-hw .. x+hw of one
+// (y, z) row hold one contiguous range of atoms (two ranges where a
+// periodic row wraps).  Range r (0 .. 2 (2 hw + 1)^2 - 1) is run r%2 of row
+// r/2; empty ranges come back as s0 == s1.
 __device__ __forceinline__ void cell_range(const Grid& g, int32_t cell, const int64_t* __restrict__ cstart, int r,
                                            int32_t& s0, int32_t& s1) {
   s0 = s1 = 0;
-  const int nx = g.nc[0];
+  const int nx = g.nc[0], hw = g.hw, span = 2 * g.hw + 1;
   const int cx = cell % nx, cy = (cell / nx) % g.nc[1], cz = cell / (nx * g.nc[1]);
-  const int nyr = g.full[1] ? g.nc[1] : 3, nzr = g.full[2] ? g.nc[2] : 3;
+  const int nyr = g.full[1] ? g.nc[1] : span, nzr = g.full[2] ? g.nc[2] : span;
   const int row = r >> 1, run = r & 1;
   if (row >= nyr * nzr) return;
   int yy = row % nyr, zz = row / nyr;
   if (!g.full[1]) {
-    yy = cy + yy - 1;
+    yy = cy + yy - hw;
     if (g.box.per[1]) yy = (yy + g.nc[1]) % g.nc[1];
     else if (yy < 0 || yy >= g.nc[1]) return;
   }
   if (!g.full[2]) {
-    zz = cz + zz - 1;
+    zz = cz + zz - hw;
     if (g.box.per[2]) zz = (zz + g.nc[2]) % g.nc[2];
     else if (zz < 0 || zz >= g.nc[2]) return;
   }
@@ -262,19 +221,19 @@ __device__ __forceinline__ void cell_range(const Grid& g, int32_t cell, const in
   int64_t a = 0, b = 0;
   if (g.full[0]) {
     if (run == 0) a = rb, b = rb + nx;
-  } else if (g.box.per[0]) {  // nx >= 3 here
-    if (cx == 0) {
-      if (run == 0) a = rb + nx - 1, b = rb + nx;
-      else a = rb, b = rb + 2;
-    } else if (cx == nx - 1) {
-      if (run == 0) a = rb + nx - 2, b = rb + nx;
-      else a = rb, b = rb + 1;
+  } else if (g.box.per[0]) {  // nx >= 2 hw + 1 here
+    if (cx - hw < 0) {
+      if (run == 0) a = rb + nx + cx - hw, b = rb + nx;
+      else a = rb, b = rb + cx + hw + 1;
+    } else if (cx + hw >= nx) {
+      if (run == 0) a = rb + cx - hw, b = rb + nx;
+      else a = rb, b = rb + cx + hw + 1 - nx;
     } else if (run == 0) {
-      a = rb + cx - 1, b = rb + cx + 2;
+      a = rb + cx - hw, b = rb + cx + hw + 1;
     }
   } else if (run == 0) {
-    a = rb + (cx > 0 ? cx - 1 : 0);
-    b = rb + (cx + 1 < nx ? cx + 1 : nx - 1) + 1;
+    a = rb + (cx - hw > 0 ? cx - hw : 0);
+    b = rb + (cx + hw < nx ? cx + hw : nx - 1) + 1;
   }
   if (b > a) {
     s0 = (int32_t)cstart[a];
@@ -329,7 +288,8 @@ struct Prefilter {
 // in cnt[i]; more than K hits raise bit 8 of *flag).
 enum { SCAN_COUNT = 0, SCAN_FILL = 1, SCAN_ELL = 2 };
 constexpr int SCAN_QCAP = 4;  // queue capacity per group, in chunks of LP
-template <int LP, int MODE, bool SC>
+constexpr int64_t HALF_CELL_MIN_N = 32768;  // atoms from which lists use half-cutoff cells
+template <int LP, int MODE, bool SC, int HW>
 __global__ __launch_bounds__(256) void k_list_scan(int64_t n, Grid g, Prefilter pf, const int32_t* __restrict__ catom,
                                                    const double4* __restrict__ spos, const float4* __restrict__ sposf,
                                                    const int32_t* __restrict__ cellof,
@@ -350,9 +310,9 @@ __global__ __launch_bounds__(256) void k_list_scan(int64_t n, Grid g, Prefilter 
   const float4 qi = sposf[k];
   int64_t w = FILL ? off[i] : 0;
   int count = 0;
-  // the 18 candidate ranges, computed up front by the group's lanes (all
-  // cstart loads in flight together) and broadcast one at a time
-  constexpr int NR = 18, NS = (NR + LP - 1) / LP;
+  // the 2 (2 HW + 1)^2 candidate ranges, computed up front by the group's
+  // lanes (all cstart loads in flight together)
+  constexpr int NR = 2 * (2 * HW + 1) * (2 * HW + 1), NS = (NR + LP - 1) / LP;
   const int32_t cell = cellof[i];
   int32_t rs[NS], re[NS];
 #pragma unroll
@@ -389,6 +349,55 @@ __global__ __launch_bounds__(256) void k_list_scan(int64_t n, Grid g, Prefilter 
     __builtin_amdgcn_wave_barrier();  // loads done before the queue is refilled
     qn = 0;
   };
+  if constexpr (HW == 2) {
+    // half-cutoff cells: 50 short ranges, walked as one concatenated list
+    // so that every chunk of LP candidates is full; per range its first
+    // slot and its offset in the list (prefix of the lengths in range order)
+    __shared__ int32_t rfirst[256 / LP][NR], roff[256 / LP][NR + 1];
+    const int grp = threadIdx.x / LP;
+    int basec = 0;
+#pragma unroll
+    for (int t = 0; t < NS; t++) {
+      const int r = l + t * LP;
+      const int len = r < NR ? re[t] - rs[t] : 0;
+      int incl = len;
+      for (int o = 1; o < LP; o <<= 1) {
+        const int y = __shfl_up(incl, o, LP);
+        if (l >= o) incl += y;
+      }
+      if (r < NR) {
+        rfirst[grp][r] = rs[t];
+        roff[grp][r] = basec + incl - len;
+      }
+      basec += __shfl(incl, LP - 1, LP);
+    }
+    if (l == 0) roff[grp][NR] = basec;
+    __builtin_amdgcn_wave_barrier();  // table stores before the loads below
+    const int total = basec;
+    int r = 0;
+    for (int c0 = 0; c0 < total; c0 += LP) {
+      const int c = c0 + l;
+      bool pass = false;
+      int32_t sl = 0;
+      if (c < total) {
+        while (roff[grp][r + 1] <= c) r++;
+        sl = rfirst[grp][r] + (c - roff[grp][r]);
+        const int32_t j = catom[sl];
+        const float4 qj = sposf[sl];
+        if (keep_pair(kind, i, j)) {
+          float dx = qj.x - qi.x, dy = qj.y - qi.y, dz = qj.z - qi.z;
+          if (g.box.per[0]) dx -= pf.L[0] * rintf(dx * pf.invL[0]);
+          if (g.box.per[1]) dy -= pf.L[1] * rintf(dy * pf.invL[1]);
+          if (g.box.per[2]) dz -= pf.L[2] * rintf(dz * pf.invL[2]);
+          pass = dx * dx + dy * dy + dz * dz <= pf.r2;
+        }
+      }
+      const uint64_t m = __ballot(pass) & gmask;
+      if (pass) qg[qn + __popcll(m & below)] = sl;
+      qn += __popcll(m);
+      if (qn > (SCAN_QCAP - 1) * LP) drain();  // room for one more chunk
+    }
+  } else
   for (int r = 0; r < NR; r++) {
     const int t = r / LP;
     int32_t a0 = rs[0], a1 = re[0];
@@ -741,9 +750,12 @@ static void extents(ipg_ctx* ctx, const DevSystem& s, double lo[3], double hi[3]
   }
 }
 
-static Grid make_grid(ipg_ctx* ctx, const DevSystem& s, double rcut) {
+// hw = 2: cells of half the cutoff, whose 5x5x5 neighborhood holds 42%
+// fewer candidates than the 3x3x3 one of full-cutoff cells.
+static Grid make_grid(ipg_ctx* ctx, const DevSystem& s, double rcut, int hw = 1) {
   Grid g{};
   g.box = s.box;
+  g.hw = hw;
   double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
   bool need = !(s.box.per[0] && s.box.per[1] && s.box.per[2]);
   if (need && s.n > 0) extents(ctx, s, lo, hi);
@@ -758,7 +770,7 @@ static Grid make_grid(ipg_ctx* ctx, const DevSystem& s, double rcut) {
       ext[d] = hi[d] - lo[d];
     }
   }
-  double cw = rcut * (1.0 + 1e-9);
+  double cw = rcut * (1.0 + 1e-9) / hw;
   int64_t cap = 4 * s.n > 64 ? 4 * s.n : 64;
   for (;;) {
     int64_t prod = 1;
@@ -774,7 +786,7 @@ static Grid make_grid(ipg_ctx* ctx, const DevSystem& s, double rcut) {
   g.ncell = 1;
   for (int d = 0; d < 3; d++) {
     g.w[d] = g.nc[d] == 1 ? (ext[d] > 0 ? ext[d] * (1.0 + 1e-12) + 1e-300 : 1.0) : ext[d] / g.nc[d];
-    g.full[d] = (s.box.per[d] && g.nc[d] < 3) ? 1 : 0;
+    g.full[d] = (s.box.per[d] && g.nc[d] < 2 * hw + 1) ? 1 : 0;
     g.ncell *= g.nc[d];
   }
   return g;
@@ -831,7 +843,8 @@ static DevList build_list_impl(ipg_ctx* ctx, const DevSystem& s, double rcut, in
     L.disp = want_disp ? (double*)alloc("disp", 8) : nullptr;
     return L;
   }
-  Grid g = make_grid(ctx, s, rcut);
+  // large systems: half-cutoff cells (the neighbor scan dominates there)
+  Grid g = make_grid(ctx, s, rcut, s.n >= HALF_CELL_MIN_N ? 2 : 1);
   const double smax = sqrt_le_threshold(rcut);
   Prefilter pf;
   double ext = 0.0;
@@ -860,7 +873,7 @@ static DevList build_list_impl(ipg_ctx* ctx, const DevSystem& s, double rcut, in
   // small periodic SNAP lists: cell-pair rows (every atom's cell scanned by
   // every cell's block: n * ncell bounded; at most 20 atoms per cell on
   // average, so the LDS lists of a cell and its neighborhood rarely fill)
-  bool packed = !binned && sc && ell_mode && ellK <= CR_KMAX && s.n <= CR_MAXN &&
+  bool packed = !binned && g.hw == 1 && sc && ell_mode && ellK <= CR_KMAX && s.n <= CR_MAXN &&
                 g.ncell <= CR_MAXCELL && s.n * g.ncell <= ((int64_t)1 << 25) && s.n <= 20 * g.ncell;
   for (int d = 0; d < 3; d++) packed = packed && s.box.per[d] && g.nc[d] >= 4 && g.nc[d] <= 1023;
   if (packed) {
@@ -931,18 +944,35 @@ static DevList build_list_impl(ipg_ctx* ctx, const DevSystem& s, double rcut, in
     });
   }
   // lanes per atom from the expected candidates per contiguous row range
-  const double cand = (double)s.n / (double)g.ncell * (g.full[0] ? g.nc[0] : std::min(3, g.nc[0]));
-  const int LP = cand >= 80 ? 64 : (cand >= 18 ? 32 : (cand >= 9 ? 16 : 8));
+  // lanes per atom: from the expected candidates per contiguous row range,
+  // or with half-cutoff cells (ranges walked as one list) per atom
+  int LP;
+  if (g.hw == 1) {
+    const double cand = (double)s.n / (double)g.ncell * (g.full[0] ? g.nc[0] : std::min(3, g.nc[0]));
+    LP = cand >= 80 ? 64 : (cand >= 18 ? 32 : (cand >= 9 ? 16 : 8));
+  } else {
+    double cand = (double)s.n / (double)g.ncell;
+    for (int d = 0; d < 3; d++) cand *= g.full[d] ? g.nc[d] : std::min(5, g.nc[d]);
+    LP = cand >= 128 ? 32 : (cand >= 48 ? 16 : 8);
+  }
   const unsigned gl = grid_for(s.n * LP, 256);
   auto scan = [&](auto lp, auto mode, int64_t* offp, int32_t* nbrp) {
     constexpr int LPc = decltype(lp)::value;
     constexpr int M = decltype(mode)::value;
-    if (sc)
-      k_list_scan<LPc, M, true><<<gl, 256, 0, ctx->stream>>>(s.n, g, pf, catom, spos, sposf, cellof, cstart, smax,
-                                                             kind, cnt, offp, nbrp, L.npairs, scd, ellK, capflag);
-    else
-      k_list_scan<LPc, M, false><<<gl, 256, 0, ctx->stream>>>(s.n, g, pf, catom, spos, sposf, cellof, cstart, smax,
-                                                              kind, cnt, offp, nbrp, L.npairs, scd, ellK, capflag);
+    auto go = [&](auto sc_c, auto hw_c) {
+      constexpr bool SCc = decltype(sc_c)::value;
+      constexpr int HWc = decltype(hw_c)::value;
+      k_list_scan<LPc, M, SCc, HWc><<<gl, 256, 0, ctx->stream>>>(s.n, g, pf, catom, spos, sposf, cellof, cstart,
+                                                                 smax, kind, cnt, offp, nbrp, L.npairs, scd, ellK,
+                                                                 capflag);
+    };
+    if (g.hw == 2) {
+      if (sc) go(std::true_type(), std::integral_constant<int, 2>());
+      else go(std::false_type(), std::integral_constant<int, 2>());
+    } else {
+      if (sc) go(std::true_type(), std::integral_constant<int, 1>());
+      else go(std::false_type(), std::integral_constant<int, 1>());
+    }
   };
   auto dispatch = [&](auto mode, int64_t* offp, int32_t* nbrp) {
     if (LP == 64) scan(std::integral_constant<int, 64>(), mode, offp, nbrp);
