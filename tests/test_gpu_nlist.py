@@ -235,3 +235,39 @@ def test_snap_list_cell_rows_overflow_falls_back(ipg, orc):
     syms = [names[k] for k in rng.integers(0, 2, len(X))]
     cnt = _snap_list_check(ipg, orc, X, L, syms, names, [1.0, 0.8], 2.4)
     assert cnt.max() > 96
+
+
+def test_snap_list_half_cells_large(ipg, orc):
+    """A CUTOFF-mode SNAP list of 40,768 atoms takes the half-cutoff-cell
+    scan with the species-pair test.  Expected rows: the oracle's cell list
+    at the query radius (PeriodicEuclidean, pinned against brute force)
+    filtered by the second test of utl.jl:84-88, r.r of the reference's
+    get_displacement against the pair's cutoff squared (the oracle's own
+    SNAP list is all-pairs and too slow at this size)."""
+    from ipgpu import workloads
+
+    rng = np.random.default_rng(8)
+    X, L = workloads.bcc_positions(28, 28, 26, 3.16)
+    X = X + rng.normal(scale=0.1, size=X.shape)
+    names = ["W", "Mo"]
+    radii, rcutfac = [1.0, 0.8], 2.4
+    spi = rng.integers(0, 2, len(X))
+    syms = [names[k] for k in spi]
+    s = ipg.AtomsSystem(X, L, (True,) * 3, syms, unit="angstrom")
+    sn = ipg.snap.SNAP(len(X), 4, names, rcutfac, 0.0, 0.99363, radii, [1.0, 1.0], list_mode="cutoff")
+    g = ipg.snap.snap_neighborlist(s, sn)
+    # cutoffs as utl.jl:42-56 forms them
+    cut = np.empty((2, 2))
+    for a in range(2):
+        for b in range(2):
+            c = 2.0 * radii[a] * rcutfac if a == b else (radii[a] + radii[b]) * rcutfac
+            cut[a, b] = c * c
+    query = max(2.0 * r * rcutfac for r in radii)  # the CUTOFF-mode query radius, cutmax
+    off, j, disp = orc.neighborlist(X, L, query, kind=orc.LIST_FULL, cells=True)
+    ci = np.repeat(np.arange(len(X)), np.diff(off))
+    rsq = (disp[:, 0] * disp[:, 0] + disp[:, 1] * disp[:, 1]) + disp[:, 2] * disp[:, 2]
+    keep = rsq <= cut[spi[ci], spi[j]]
+    cnt = np.bincount(ci[keep], minlength=len(X))
+    np.testing.assert_array_equal(np.diff(g.offsets), cnt)
+    np.testing.assert_array_equal(g.j, j[keep])
+    np.testing.assert_array_equal(g.r, -disp[keep])
