@@ -137,6 +137,8 @@ struct Dual {
   __host__ __device__ Dual(T a, T b) : v(a), t(b) {}
   template <typename U, typename = typename std::enable_if<std::is_arithmetic<U>::value && !std::is_same<U, T>::value>::type>
   __host__ __device__ explicit Dual(U a) : v((T)a), t(0) {}
+  template <typename U, typename = typename std::enable_if<!std::is_same<U, T>::value>::type>
+  __host__ __device__ explicit Dual(Dual<U> a) : v((T)a.v), t((T)a.t) {}
 };
 template <typename T> __device__ __forceinline__ Dual<T> operator+(Dual<T> a, Dual<T> b) { return {a.v + b.v, a.t + b.t}; }
 template <typename T> __device__ __forceinline__ Dual<T> operator-(Dual<T> a, Dual<T> b) { return {a.v - b.v, a.t - b.t}; }

@@ -820,7 +820,7 @@ static bool energy_forces_dev(ipg_snap* sn, const SnapCall& c, const double* d_b
       snap_launch_ycoef<double>(ctx, P, d_beta, yc);
       snap_launch_ui<double>(ctx, P, c.R, U);
       snap_launch_zby<double>(ctx, P, c.R, U, d_beta, yc, nullptr, eatom, Y, nullptr, nullptr);
-      snap_launch_dedr<double>(ctx, P, c.R, Y, dedr, nullptr);
+      snap_launch_dedr_rev<double>(ctx, P, c.R, Y, dedr, nullptr);
     } else if (path == IPG_SNAP_ADJOINT_FP32) {
       const size_t nblk32 = (size_t)((n + 31) / 32) * 32;  // atom-blocked layout, padded to whole batches
       float* U = scratch_t<float>(ctx, "snap_U32", 2 * (size_t)P.idxu_max * nblk32);
@@ -1307,7 +1307,7 @@ int ipg_snap_force_vjp(ipg_snap* sn, const ipg_system* sys, int mode, const int3
       snap_launch_ycoef<double>(ctx, P, db, yc);
       snap_launch_ui<S>(ctx, P, c.R, U);
       snap_launch_zby<S>(ctx, P, c.R, U, nullptr, yc, nullptr, nullptr, Y, nullptr, tanB);
-      snap_launch_dedr<S>(ctx, P, c.R, Y, dedr, dedr_t);
+      snap_launch_dedr_rev<S>(ctx, P, c.R, Y, dedr, dedr_t);
     } else {
       using S = Dual<float>;
       S* U = scratch_t<S>(ctx, "snap_Ud32", 2 * (size_t)P.idxu_max * n);
@@ -1316,7 +1316,7 @@ int ipg_snap_force_vjp(ipg_snap* sn, const ipg_system* sys, int mode, const int3
       snap_launch_ycoef<float>(ctx, P, db, yc);
       snap_launch_ui<S>(ctx, P, c.R, U);
       snap_launch_zby<S>(ctx, P, c.R, U, nullptr, yc, nullptr, nullptr, Y, nullptr, tanB);
-      snap_launch_dedr<S>(ctx, P, c.R, Y, dedr, dedr_t);
+      snap_launch_dedr_rev<S>(ctx, P, c.R, Y, dedr, dedr_t);
     }
     double* gp = scratch_t<double>(ctx, "snap_vjp_gpos", 3 * n);
     force_gather(ctx, c.R, dedr_t, gp);

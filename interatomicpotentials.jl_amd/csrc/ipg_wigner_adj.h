@@ -95,20 +95,38 @@ struct AB {
 };
 
 // Phi and the adjoints of a and b.  Each level's partial sums (in the
-// chain's scalar type) are added into FP64 once.
-struct Acc {
-  double phi = 0.0, ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0;
+// chain's scalar type) are added into the wider type W once: FP64, or a
+// dual number over FP64 for a sweep run on dual numbers.
+template <typename W>
+struct AccW {
+  W phi = W(0.0), ar = W(0.0), ai = W(0.0), br = W(0.0), bi = W(0.0);
+};
+using Acc = AccW<double>;
+
+// Scalar type under T: T itself, specialised for dual numbers where they
+// are defined.
+template <typename T>
+struct Base {
+  using type = T;
 };
 
 // Keeps an accumulator update where it is written: otherwise the compiler
 // sinks the FP64 additions to the end of the sweep and every partial sum
-// stays live until then.
+// stays live until then.  (Accumulators of other types supply their own
+// pin, found by argument-dependent lookup.)
 __host__ __device__ __forceinline__ void pin(double& x) {
 #if defined(__HIP_DEVICE_COMPILE__)
   asm volatile("" : "+v"(x));
 #else
   (void)x;
 #endif
+}
+
+// dst += x in the accumulator's type, pinned where it is written.
+template <typename W, typename T>
+__host__ __device__ __forceinline__ void acc_add(W& dst, const T& x) {
+  dst += W(x);
+  pin(dst);
 }
 
 // Level JJ of row MB from level JJ-1 (JJ entries in, JJ+1 out), rescaled.
@@ -140,10 +158,10 @@ __host__ __device__ __forceinline__ void fstep(const AB<T>& c, const T (&pr)[JJ]
 // LMAX are zero and skipped), qr/qi receives the adjoint of level JJ-1 and
 // acc the adjoints of a and b.  Returns the step's partial sum of Re(a_bar),
 // which depends on every entry of the step (an ordering handle).
-template <typename T, int JJ, int MB, int LMAX>
+template <typename T, int JJ, int MB, int LMAX, typename A>
 __host__ __device__ __forceinline__ T rstep(const AB<T>& c, const T (&pr)[JJ], const T (&pi)[JJ],
                                                const T (&lr)[JJ + 1], const T (&li)[JJ + 1],
-                                               T (&qr)[JJ], T (&qi)[JJ], Acc& acc) {
+                                               T (&qr)[JJ], T (&qi)[JJ], A& acc) {
   T abr = (T)0, abi = (T)0, bbr = (T)0, bbi = (T)0;
   sfor<0, LMAX + 1>([&](auto mc) {
     constexpr int ma = decltype(mc)::value;
@@ -169,14 +187,10 @@ __host__ __device__ __forceinline__ T rstep(const AB<T>& c, const T (&pr)[JJ], c
       bbi = bbi + pr[ma - 1] * vi;
     }
   });
-  acc.ar += (double)abr;
-  acc.ai += (double)abi;
-  acc.br += (double)bbr;
-  acc.bi += (double)bbi;
-  pin(acc.ar);
-  pin(acc.ai);
-  pin(acc.br);
-  pin(acc.bi);
+  acc_add(acc.ar, abr);
+  acc_add(acc.ai, abi);
+  acc_add(acc.br, bbr);
+  acc_add(acc.bi, bbi);
   return abr;
 }
 
@@ -235,10 +249,10 @@ __host__ __device__ __forceinline__ void stair(const AB<T>& c, T (&sr)[2 * MB], 
 // completed backward step and uses them after its own backward step, so the
 // reads overlap one step of arithmetic and no level's seeds are held longer.
 // done returns a value that depends on this level's completed step.
-template <typename T, int J, int MB, int JJ, typename Seed>
+template <typename T, int J, int MB, int JJ, typename Seed, typename A>
 __host__ __device__ __forceinline__ void level(const AB<T>& c, const T (&vr)[JJ + 1], const T (&vi)[JJ + 1],
                                                T (&lr)[JJ + 1], T (&li)[JJ + 1],
-                                               const T (&incr)[J + 1], const T (&inci)[J + 1], Acc& acc,
+                                               const T (&incr)[J + 1], const T (&inci)[J + 1], A& acc,
                                                Seed& seed, T& done) {
   // deeper levels first: on the way up only the values of the levels below
   // stay live
@@ -276,24 +290,24 @@ __host__ __device__ __forceinline__ void level(const AB<T>& c, const T (&vr)[JJ 
     constexpr int ma = decltype(mc)::value;
     if constexpr (JJ > 2 * MB || ma <= MB) {
       // 2 w(e) N(e), unless the seed comes with it (Seed::weighted)
-      constexpr T wn = Seed::weighted ? (T)1 : (T)seed_weight(JJ, MB, ma);
-      const T gr = wn * sr[ma], gi = wn * si[ma];
+      using B = typename Base<T>::type;
+      constexpr B wn = Seed::weighted ? (B)1 : (B)seed_weight(JJ, MB, ma);
+      const T gr = sr[ma] * wn, gi = si[ma] * wn;
       lr[ma] = lr[ma] + gr;
       li[ma] = li[ma] + gi;
       ph = ph + vr[ma] * gr;
       ph = ph + vi[ma] * gi;
     }
   });
-  acc.phi += (double)ph;
-  pin(acc.phi);
+  acc_add(acc.phi, ph);
   if constexpr (JJ == J) done = ph;
 }
 
 // Row MB: forward and backward sweep.  inc holds, on entry, the adjoint that
 // row MB+1 handed to level 2MB+1 of this row, and on return the adjoint this
 // row hands to row MB-1 at level 2MB-1.
-template <typename T, int J, int MB, typename Seed>
-__host__ __device__ __forceinline__ void row(const AB<T>& c, T (&incr)[J + 1], T (&inci)[J + 1], Acc& acc,
+template <typename T, int J, int MB, typename Seed, typename A>
+__host__ __device__ __forceinline__ void row(const AB<T>& c, T (&incr)[J + 1], T (&inci)[J + 1], A& acc,
                                              Seed& seed) {
   if constexpr (MB == 0) {
     const T vr[1] = {(T)1}, vi[1] = {(T)0};
@@ -345,8 +359,8 @@ __host__ __device__ __forceinline__ void row(const AB<T>& c, T (&incr)[J + 1], T
 // depends on x (0 where the order does not matter) and seed.tie(v, x)
 // returns v made to depend on x.  Seed::weighted (constexpr bool) says that
 // the seeds already carry their weights seed_weight(j, mb, ma).
-template <typename T, int J, typename Seed>
-__host__ __device__ __forceinline__ void vjp(const AB<T>& c, Seed& seed, Acc& acc) {
+template <typename T, int J, typename Seed, typename A>
+__host__ __device__ __forceinline__ void vjp(const AB<T>& c, Seed& seed, A& acc) {
   T incr[J + 1], inci[J + 1];
   sfor<0, J + 1>([&](auto xc) {
     incr[decltype(xc)::value] = (T)0;

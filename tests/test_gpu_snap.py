@@ -73,7 +73,7 @@ def test_reference_mode_zero_neighbors(ipg, orc):
     assert rel(B, Bo) < 1e-12
 
 
-@pytest.mark.parametrize("J", [2, 6, 8])
+@pytest.mark.parametrize("J", [0, 1, 2, 3, 4, 5, 6, 7, 8])
 def test_energy_forces_paths(ipg, orc, J):
     s = system(ipg, seed=7)
     g, o, sp, (off, nbr, r) = make(ipg, orc, s, J, switch=True)
@@ -103,10 +103,13 @@ def test_shared_beta_matches_reference_energy(ipg, orc):
     assert abs(e - Bo.sum(0) @ beta) <= 1e-10 * abs(e)  # linear_potential.jl:20-26
 
 
+@pytest.mark.parametrize("J", [0, 1, 2, 3, 4, 5, 6, 7, 8])
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-9), ("fp32", 2e-4)])
-def test_force_vjp(ipg, orc, precision, tol):
+def test_force_vjp(ipg, orc, precision, tol, J):
+    """The pullback's dual force kernel is compiled per twojmax: gbeta
+    against the oracle's dB and gpos against central differences of
+    gF . F at every level."""
     s = system(ipg, seed=9)
-    J = 6
     g, o, sp, (off, nbr, r) = make(ipg, orc, s, J, switch=True)
     rng = np.random.default_rng(2)
     beta = rng.normal(scale=0.1, size=2 * g.ncoeff)
