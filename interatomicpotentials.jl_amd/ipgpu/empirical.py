@@ -15,6 +15,7 @@ import numpy as np
 from . import _lib
 from ._lib import IPG_FP32, IPG_FP64, check
 from .nnlist import make_csystem
+from .system import symbol_codes
 
 EnergyForce = collections.namedtuple("EnergyForce", ["e", "f"])
 
@@ -200,10 +201,9 @@ def _eval(s, terms, precision):
     """One fused GPU evaluation of (potential, coefficient) terms."""
     L = _lib.load()
     ctx = _lib.default_context()
-    table = s.unique_symbols()
+    table, sp = symbol_codes(s)
     if len(table) > 64:
         raise ValueError("more than 64 species")
-    sp = np.array([table.index(x) for x in s.symbols], dtype=np.int32)
     cs, keep = make_csystem(s.austrip_positions(), s.austrip_box(), s.periodic, sp)
     arr = (_lib.PairTerm * len(terms))()
     for k, (p, c) in enumerate(terms):
@@ -273,8 +273,7 @@ class VerletEvaluator:
 
         L = _lib.load()
         self._ctx = _lib.default_context()
-        self.table = s.unique_symbols()
-        sp = np.array([self.table.index(x) for x in s.symbols], dtype=np.int32)
+        self.table, sp = symbol_codes(s)
         terms = _terms_of(p)
         self._arr = (_lib.PairTerm * len(terms))()
         for k, (q, c) in enumerate(terms):

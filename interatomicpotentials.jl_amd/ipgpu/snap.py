@@ -14,7 +14,7 @@ from . import _lib
 from ._lib import (IPG_FP32, IPG_FP64, IPG_SNAP_ADJOINT_FP32, IPG_SNAP_ADJOINT_FP64, IPG_SNAP_LIST_CUTOFF,
                    IPG_SNAP_LIST_REFERENCE, IPG_SNAP_PLAIN_FP64, check)
 from .nnlist import make_csystem
-from .system import species_index
+from .system import species_index, symbol_codes
 
 EnergyForce = collections.namedtuple("EnergyForce", ["e", "f"])
 SnapNeighborList = collections.namedtuple("SnapNeighborList", ["offsets", "j", "R", "r"])
@@ -105,10 +105,9 @@ class SNAP:
 
     # helpers ------------------------------------------------------------
     def _sys(self, s):
-        sp = species_index(s.symbols, self.species)
+        uniq, appear = symbol_codes(s)
+        sp = species_index(uniq, self.species)[appear]
         cs, keep = make_csystem(s.positions, s.box, s.periodic, sp)
-        uniq = s.unique_symbols()
-        appear = np.array([uniq.index(x) for x in s.symbols], dtype=np.int32)
         return cs, keep, appear
 
     def _mode(self, list_mode):
@@ -188,7 +187,8 @@ def compute_batch_descriptors(systems, snap, precision="fp64", want=("B", "dB", 
     arr = (_lib.System * len(systems))()
     keep = []
     for c, s in enumerate(systems):
-        cs, k = make_csystem(s.positions, s.box, s.periodic, species_index(s.symbols, snap.species))
+        uniq, codes = symbol_codes(s)
+        cs, k = make_csystem(s.positions, s.box, s.periodic, species_index(uniq, snap.species)[codes])
         arr[c] = cs
         keep.append(k)
     natoms = np.array([a.natoms for a in arr], dtype=np.int64)

@@ -53,6 +53,28 @@ class AtomsSystem:
         self.symbols = syms
         self.unit = unit
 
+    # symbols are kept as a tuple, so the per-atom species codes derived
+    # from them can be formed once per system
+    @property
+    def symbols(self):
+        return self._symbols
+
+    @symbols.setter
+    def symbols(self, v):
+        self._symbols = tuple(str(x) for x in v)
+        self._codes = None
+
+    def symbol_codes(self):
+        """(unique_symbols(), the index of every atom's symbol in that list
+        as a read-only int32 array), formed on first use."""
+        if self._codes is None:
+            first = {}
+            codes = np.fromiter((first.setdefault(x, len(first)) for x in self._symbols), dtype=np.int32,
+                                count=len(self._symbols))
+            codes.flags.writeable = False
+            self._codes = (tuple(first), codes)
+        return list(self._codes[0]), self._codes[1]
+
     def __len__(self):
         return self.positions.shape[0]
 
@@ -83,6 +105,17 @@ def periodic_system(atoms, box, unit="bohr"):
     syms = [a[0] for a in atoms]
     pos = [a[1] for a in atoms]
     return AtomsSystem(pos, box, (True, True, True), syms, unit)
+
+
+def symbol_codes(s):
+    """(unique symbols in first-appearance order, int32 index of every
+    atom's symbol among them) of a system; cached by AtomsSystem."""
+    f = getattr(s, "symbol_codes", None)
+    if f is not None:
+        return f()
+    uniq = s.unique_symbols()
+    lut = {x: i for i, x in enumerate(uniq)}
+    return uniq, np.array([lut[x] for x in s.symbols], dtype=np.int32)
 
 
 def species_index(symbols, table):
