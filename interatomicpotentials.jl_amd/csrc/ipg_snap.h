@@ -190,8 +190,6 @@ template <typename S>
 void snap_launch_zby(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const S* ulisttot, const double* beta,
                      const typename BaseOf<S>::type* ycoef, double* Bout, double* eatom, S* Yout, S* Zout,
                      double* tanB);
-template <typename S>
-void snap_launch_dedr(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const S* Y, double* dedr, double* dedr_t);
 template <typename T>
 void snap_launch_ycoef(ipg_ctx* ctx, const SnapDev& P, const double* beta, T* ycoef);
 // Per-pair dB_i/dr_ij [P][ncoeff][3] from the center atoms' Z [N][idxz_max][2].

@@ -726,68 +726,6 @@ __global__ __launch_bounds__(ZBY_THREADS) void k_snap_zby(SnapDev P, SnapRun R, 
   }
 }
 
-// ---------------------------------------------------------------------------
-// K3: adjoint force per pair.  dE_i/dr_ij = 2 w_j sum_half w(e)
-// Re(conj(dsfac u e_r + sfac du) Y_i), one thread per pair, the u/du chain
-// in registers (no per-pair dB is formed).
-// ---------------------------------------------------------------------------
-template <int J, typename S>
-__global__ __launch_bounds__(256) void k_snap_dedr(SnapDev P, SnapRun R, const S* __restrict__ Y,
-                                                   double* __restrict__ dedr, double* __restrict__ dedr_t) {
-  using T = typename BaseOf<S>::type;
-  constexpr int IDXU = ub_of(J + 1);
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= (R.d_np ? *R.d_np : R.npairs)) return;
-  const int32_t i = R.center[q], j = R.nbr[q];
-  const int si = snap_species(R.species, i, P.nspecies), sj = snap_species(R.species, j, P.nspecies);
-  typename WideOf<S>::type x, y, z;
-  load_pair<typename WideOf<S>::type>(R, q, i, j, x, y, z);
-  const double rcut = (P.radii[si] + P.radii[sj]) * P.rcutfac;
-  CK<S> c;
-  cayley_klein<S, true>(x, y, z, rcut, P.weight[sj], P, c);
-  // FP32 runs read the atom-blocked Y of the Z/Y kernel
-  constexpr bool BLK = std::is_same<S, float>::value;
-  constexpr int ES = BLK ? 64 : 2;  // element stride
-  const S* Yi = BLK ? Y + (((int64_t)i >> 5) * IDXU * 32 + (i & 31)) * 2 : Y + (int64_t)i * IDXU * 2;
-  // per-row partial sums in S, flushed into A (FP64 for FP32 runs) at the
-  // end of every emitted row
-  using A = typename AccOf<S>::type;
-  A a0 = A(0.0), a1 = A(0.0), a2 = A(0.0), a3 = A(0.0);
-  S s0 = S((T)0), s1 = S((T)0), s2 = S((T)0), s3 = S((T)0);
-  wigner_chain<J, S, 3>(c, [&](auto jc, auto mbc, auto mac, auto hc, auto lc, S u_r, S u_i, auto& dur, auto& dui) {
-    constexpr int jl = decltype(jc)::value, mb = decltype(mbc)::value, ma = decltype(mac)::value;
-    constexpr int jju = ub_of(jl) + (jl + 1) * mb + ma;
-    S yr = Yi[ES * jju], yi = Yi[ES * jju + 1];
-    if constexpr (decltype(hc)::value == 1) {
-      yr = (T)0.5 * yr;
-      yi = (T)0.5 * yi;
-    }
-    s0 += u_r * yr + u_i * yi;
-    s1 += dur[0][ma] * yr + dui[0][ma] * yi;
-    s2 += dur[1][ma] * yr + dui[1][ma] * yi;
-    s3 += dur[2][ma] * yr + dui[2][ma] * yi;
-    if constexpr (decltype(lc)::value == 1) {
-      a0 += to_acc(s0);
-      a1 += to_acc(s1);
-      a2 += to_acc(s2);
-      a3 += to_acc(s3);
-      s0 = s1 = s2 = s3 = S((T)0);
-    }
-  });
-  const A g0 = to_acc(c.dsfac) * a0, sf = to_acc(c.sfac);
-  const A d0 = 2.0 * (g0 * to_acc(c.u[0]) + sf * a1);
-  const A d1 = 2.0 * (g0 * to_acc(c.u[1]) + sf * a2);
-  const A d2 = 2.0 * (g0 * to_acc(c.u[2]) + sf * a3);
-  dedr[3 * q] = value_of(d0);
-  dedr[3 * q + 1] = value_of(d1);
-  dedr[3 * q + 2] = value_of(d2);
-  if (dedr_t) {
-    dedr_t[3 * q] = tangent_of(d0);
-    dedr_t[3 * q + 1] = tangent_of(d1);
-    dedr_t[3 * q + 2] = tangent_of(d2);
-  }
-}
-
 template <typename T>
 __global__ void k_snap_ycoef(SnapDev P, const double* __restrict__ beta, T* __restrict__ ycoef) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -854,42 +792,16 @@ void snap_launch_zby(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const S* 
   });
 }
 
-template <typename S>
-void snap_launch_dedr(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const S* Y, double* dedr, double* dedr_t) {
-  if (R.npairs == 0) return;
-  unsigned g = grid_for(R.npairs, 256);
-  const char* nm = IsDual<S>::value ? "snap_dedr_dual" : (sizeof(S) == 4 ? "snap_dedr_fp32" : "snap_dedr_fp64");
-  launch(ctx, nm, [&] {
-    switch (P.J) {
-      case 0: k_snap_dedr<0, S><<<g, 256, 0, ctx->stream>>>(P, R, Y, dedr, dedr_t); break;
-      case 1: k_snap_dedr<1, S><<<g, 256, 0, ctx->stream>>>(P, R, Y, dedr, dedr_t); break;
-      case 2: k_snap_dedr<2, S><<<g, 256, 0, ctx->stream>>>(P, R, Y, dedr, dedr_t); break;
-      case 3: k_snap_dedr<3, S><<<g, 256, 0, ctx->stream>>>(P, R, Y, dedr, dedr_t); break;
-      case 4: k_snap_dedr<4, S><<<g, 256, 0, ctx->stream>>>(P, R, Y, dedr, dedr_t); break;
-      case 5: k_snap_dedr<5, S><<<g, 256, 0, ctx->stream>>>(P, R, Y, dedr, dedr_t); break;
-      case 6: k_snap_dedr<6, S><<<g, 256, 0, ctx->stream>>>(P, R, Y, dedr, dedr_t); break;
-      case 7: k_snap_dedr<7, S><<<g, 256, 0, ctx->stream>>>(P, R, Y, dedr, dedr_t); break;
-      default: k_snap_dedr<8, S><<<g, 256, 0, ctx->stream>>>(P, R, Y, dedr, dedr_t); break;
-    }
-  });
-}
-
 #define IPG_SNAP_INSTANTIATE_U(S) template void snap_launch_ui<S>(ipg_ctx*, const SnapDev&, const SnapRun&, S*);
-#define IPG_SNAP_INSTANTIATE_UD(S)                                                                               \
-  template void snap_launch_ui<S>(ipg_ctx*, const SnapDev&, const SnapRun&, S*);                                 \
-  template void snap_launch_dedr<S>(ipg_ctx*, const SnapDev&, const SnapRun&, const S*, double*, double*);
 #define IPG_SNAP_INSTANTIATE(S)                                                                                  \
-  IPG_SNAP_INSTANTIATE_UD(S)                                                                                     \
+  IPG_SNAP_INSTANTIATE_U(S)                                                                                      \
   template void snap_launch_zby<S>(ipg_ctx*, const SnapDev&, const SnapRun&, const S*, const double*,            \
                                    const typename BaseOf<S>::type*, double*, double*, S*, S*, double*);
 
 // Each launcher is instantiated once, in ipg_snap_k_*.hip; every other
 // source that includes this header only calls them.
-#define IPG_SNAP_EXTERN_UD(S)                                                                                    \
-  extern template void snap_launch_ui<S>(ipg_ctx*, const SnapDev&, const SnapRun&, S*);                          \
-  extern template void snap_launch_dedr<S>(ipg_ctx*, const SnapDev&, const SnapRun&, const S*, double*, double*);
 #define IPG_SNAP_EXTERN(S)                                                                                       \
-  IPG_SNAP_EXTERN_UD(S)                                                                                          \
+  extern template void snap_launch_ui<S>(ipg_ctx*, const SnapDev&, const SnapRun&, S*);                          \
   extern template void snap_launch_zby<S>(ipg_ctx*, const SnapDev&, const SnapRun&, const S*, const double*,     \
                                           const typename BaseOf<S>::type*, double*, double*, S*, S*, double*);
 extern template void snap_launch_ui<float>(ipg_ctx*, const SnapDev&, const SnapRun&, float*);
