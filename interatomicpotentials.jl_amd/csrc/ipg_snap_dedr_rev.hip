@@ -67,8 +67,15 @@ struct PlainYSeed {
   }
 };
 
+// Waves per SIMD: two in FP64 and on FP32 duals; FP64 duals need the
+// whole register file of one.
+template <typename S>
+constexpr int rev_waves() {
+  return std::is_same<S, Dual<double>>::value ? 1 : 2;
+}
+
 template <int J, typename S>
-__global__ __launch_bounds__(256) void k_snap_dedr_rev(SnapDev P, SnapRun R, const S* __restrict__ Y,
+__global__ __launch_bounds__(256, rev_waves<S>()) void k_snap_dedr_rev(SnapDev P, SnapRun R, const S* __restrict__ Y,
                                                        double* __restrict__ dedr, double* __restrict__ dedr_t) {
   using W = typename WideOf<S>::type;  // double, or Dual<double>
   constexpr int IDXU = ub_of(J + 1);
