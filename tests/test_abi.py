@@ -51,6 +51,37 @@ def test_units_and_system():
     assert s2.unique_symbols() == ["B", "A", "C"]
 
 
+def test_symbol_codes():
+    """The per-atom species codes the wrappers pass to the library: each
+    atom's index among the unique symbols in first-appearance order, formed
+    once per system and redone when the symbols are replaced."""
+    import ipgpu
+    from ipgpu.system import species_index, symbol_codes
+
+    s = ipgpu.AtomsSystem(np.zeros((6, 3)), [1, 1, 1], (1, 1, 1), ["W", "Mo", "Mo", "W", "Ta", "Mo"])
+    uniq, codes = symbol_codes(s)
+    assert uniq == s.unique_symbols() == ["W", "Mo", "Ta"]
+    assert codes.dtype == np.int32 and codes.tolist() == [0, 1, 1, 0, 2, 1]
+    assert symbol_codes(s)[1] is codes  # cached
+    with pytest.raises(ValueError):
+        codes[0] = 1  # read-only
+    # species table order differs from first appearance: the table index per atom
+    assert species_index(uniq, ["Mo", "Ta", "W"])[codes].tolist() == species_index(s.symbols, ["Mo", "Ta", "W"]).tolist()
+    with pytest.raises(ValueError):
+        species_index(uniq, ["W", "Mo"])  # Ta is not in the table
+    s.symbols = ["Ta"] * 6
+    assert symbol_codes(s)[0] == ["Ta"] and symbol_codes(s)[1].tolist() == [0] * 6
+
+    class Duck:  # any system with symbols and unique_symbols()
+        symbols = ["B", "A", "B"]
+
+        def unique_symbols(self):
+            return ["B", "A"]
+
+    u2, c2 = symbol_codes(Duck())
+    assert u2 == ["B", "A"] and c2.tolist() == [0, 1, 0]
+
+
 def test_linear_combination_algebra_and_terms():
     import ipgpu
 
