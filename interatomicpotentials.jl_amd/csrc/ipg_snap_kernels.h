@@ -592,11 +592,22 @@ __global__ __launch_bounds__(64) void k_snap_ui(SnapDev P, SnapRun R, S* __restr
 // Y_l(e) = sum over Z blocks (a,c,l) of coef(a,c,l) Z_(a,c,l)(e), with coef
 // collecting the beta entries whose dB/dr half-sums use that block.
 // ---------------------------------------------------------------------------
-constexpr int ZBY_THREADS = 256;
-constexpr int ZBY_KMAX = 10;  // ceil(idxz_max / 256) at 2J = 8
+// Block shape: 256 threads, each with up to 10 Z entries (2,386 at
+// 2J = 8); FP64 dual numbers take 512 threads with 5 entries each, since
+// their U and Z in LDS (85 KB) leave room for one block per CU (measured
+// 1.14 ms against 1.47 ms with 256; FP32 duals, three blocks per CU, were
+// slower at 512).
+template <typename S>
+struct ZbyShape {
+  static constexpr int NT = 256, KMAX = 10;
+};
+template <>
+struct ZbyShape<Dual<double>> {
+  static constexpr int NT = 512, KMAX = 5;
+};
 
 template <typename S>
-__global__ __launch_bounds__(ZBY_THREADS) void k_snap_zby(SnapDev P, SnapRun R, const S* __restrict__ ulisttot,
+__global__ __launch_bounds__(ZbyShape<S>::NT) void k_snap_zby(SnapDev P, SnapRun R, const S* __restrict__ ulisttot,
                                                           const double* __restrict__ beta,
                                                           const typename BaseOf<S>::type* __restrict__ ycoef,
                                                           double* __restrict__ Bout, double* __restrict__ eatom,
@@ -604,6 +615,7 @@ __global__ __launch_bounds__(ZBY_THREADS) void k_snap_zby(SnapDev P, SnapRun R, 
                                                           double* __restrict__ tanB) {
   using T = typename BaseOf<S>::type;
   using A = typename AccOf<S>::type;
+  constexpr int ZBY_THREADS = ZbyShape<S>::NT, ZBY_KMAX = ZbyShape<S>::KMAX;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* cgL = reinterpret_cast<T*>(smem_raw);
   S* U = reinterpret_cast<S*>(smem_raw + (((size_t)P.idxcg_max * sizeof(T) + 15) & ~(size_t)15));
@@ -705,7 +717,11 @@ __global__ __launch_bounds__(ZBY_THREADS) void k_snap_zby(SnapDev P, SnapRun R, 
       for (int o = 32; o > 0; o >>= 1) epart += __shfl_xor(epart, o, 64);
       if ((tid & 63) == 0) red[tid >> 6] = epart;
       __syncthreads();
-      if (tid == 0) eatom[a] = ((red[0] + red[1]) + red[2]) + red[3];
+      if (tid == 0) {
+        double t = red[0];
+        for (int w = 1; w < ZBY_THREADS / 64; w++) t += red[w];
+        eatom[a] = t;
+      }
     }
     if (Yout) {
       S* yg = Yout + a * (int64_t)P.idxu_max * 2;
@@ -778,7 +794,8 @@ void snap_launch_zby(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const S* 
                      double* tanB) {
   using T = typename BaseOf<S>::type;
   if (R.n == 0) return;
-  IPG_REQUIRE(P.idxz_max <= ZBY_THREADS * ZBY_KMAX, "internal: Z table larger than the kernel plan");
+  constexpr int NT = ZbyShape<S>::NT;
+  IPG_REQUIRE(P.idxz_max <= NT * ZbyShape<S>::KMAX, "internal: Z table larger than the kernel plan");
   const size_t sm = (((size_t)P.idxcg_max * sizeof(T) + 15) & ~(size_t)15) +
                     sizeof(S) * 2 * (size_t)(P.idxu_max + P.idxz_max);
   const int per_cu = std::max(1, std::min(8, (int)((160 * 1024) / (sm + 1024))));
@@ -787,8 +804,8 @@ void snap_launch_zby(ipg_ctx* ctx, const SnapDev& P, const SnapRun& R, const S* 
   launch(ctx, nm, [&] {
     if (sm > 64 * 1024)
       IPG_HIP(hipFuncSetAttribute((const void*)k_snap_zby<S>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm));
-    k_snap_zby<S><<<(unsigned)grid, ZBY_THREADS, sm, ctx->stream>>>(P, R, ulisttot, beta, ycoef, Bout, eatom, Yout,
-                                                                    Zout, tanB);
+    k_snap_zby<S><<<(unsigned)grid, NT, sm, ctx->stream>>>(P, R, ulisttot, beta, ycoef, Bout, eatom, Yout, Zout,
+                                                           tanB);
   });
 }
 
