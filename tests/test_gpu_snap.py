@@ -137,6 +137,21 @@ def test_force_vjp(ipg, orc, precision, tol, J):
         assert abs(gp[a, k] - fd) <= max(tol, 1e-6) * max(np.abs(gp).max(), 1.0), (a, k, gp[a, k], fd)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_force_vjp_without_pairs(ipg, precision):
+    """Atoms farther apart than every cutoff: no pairs, so F = 0 for every
+    beta and position and the pullback is zero (the force kernels launch
+    nothing and the gathers read empty rows)."""
+    s = ipg.AtomsSystem(np.array([[1.0, 1.0, 1.0], [9.0, 9.0, 9.0], [1.0, 9.0, 5.0]]), [20.0, 20.0, 20.0],
+                        (True, True, True), ["W", "Mo", "W"], unit="angstrom")
+    g = ipg.snap.SNAP(len(s), 6, ["W", "Mo"], 2.5, 0.0, 0.99363, [1.0, 1.0], [1.0, 0.8], switch_flag=True,
+                      list_mode="cutoff")
+    beta = np.random.default_rng(4).normal(scale=0.1, size=2 * g.ncoeff)
+    lbp = ipg.snap.LBasisPotential(beta, [0.0], g)
+    gb, gp = ipg.snap.force_vjp(s, lbp, np.ones((len(s), 3)), precision=precision)
+    assert np.all(gb == 0.0) and np.all(gp == 0.0)
+
+
 @pytest.mark.parametrize("J", [0, 1, 2, 3, 4, 5, 6, 7, 8])
 @pytest.mark.parametrize("bzero", [False, True])
 def test_adjoint_fp32_every_level(ipg, orc, J, bzero):
