@@ -480,7 +480,7 @@ def probe():
     writes COPY_BYTES), so FETCH_SIZE / WRITE_SIZE of the SNAP kernels can be
     scaled by what the counters read for it."""
     import ipgpu
-    from ipgpu._lib import IPG_SNAP_ADJOINT_FP32, IPG_SNAP_PLAIN_FP64, check, default_context
+    from ipgpu._lib import IPG_SNAP_ADJOINT_FP32, IPG_SNAP_ADJOINT_FP64, IPG_SNAP_PLAIN_FP64, check, default_context
     from ipgpu.device import DeviceArray, set_graphs, synchronize
 
     src = DeviceArray((COPY_BYTES // 8,), np.float64)
@@ -494,7 +494,13 @@ def probe():
     for _ in range(3):
         snap_step(sn, dsys, d, IPG_SNAP_ADJOINT_FP32)
     snap_step(sn, dsys, d, IPG_SNAP_PLAIN_FP64)
+    snap_step(sn, dsys, d, IPG_SNAP_ADJOINT_FP64)
     synchronize()
+    # one FP64 and one FP32 force pullback on the same workload
+    lbp = ipgpu.snap.LBasisPotential(beta, 0.0, sn)
+    gF = np.random.default_rng(57).normal(size=(len(s), 3))
+    for prec in ("fp64", "fp32"):
+        ipgpu.snap.force_vjp(s, lbp, gF, precision=prec)
     # one uncaptured FP32 batched-descriptor call over 32 configurations
     from ipgpu import workloads
 
@@ -546,7 +552,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=2048, help="atoms of the 1-thread CPU baseline")
     ap.add_argument("--probe", action="store_true",
                     help="short run for hardware-counter passes: 3 calibration copies of 256 MiB, 3 uncaptured "
-                         "SNAP FP32 steps, one FP64 plain step, one FP32 batched-descriptor call (32 configurations)")
+                         "SNAP FP32 steps, one FP64 plain and one FP64 adjoint step, one FP64 and one FP32 force "
+                         "pullback, one FP32 batched-descriptor call (32 configurations), six pair-kernel calls")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: every box here has one GPU and this engine has no multi-GPU path "
